@@ -1,0 +1,283 @@
+/*
+ * karam_nr.h — C ABI of the MI355X Newton-Raphson inner loop.
+ *
+ * Drop-in boundary for the reference engine's NR hot path.  Every entry point
+ * is `extern "C"`, takes plain pointers and sizes, returns an int status
+ * (NR_OK == 0) and never lets an exception cross the boundary.  The exported
+ * symbols carry default visibility (the reference's ENGINE_API convention,
+ * engine_export.h:19-39); everything else in the library is hidden.
+ *
+ * Conventions (mirroring the reference):
+ *   - node ids are 1-based, 0 is ground; `num_nodes` is numGlobalNodes
+ *     (netlist/circuit.h:1056-1267);
+ *   - voltages vectors are 0-based, index k holds node k+1;
+ *   - component arrays are given in netlist order (the order the caller
+ *     added them); the library derives the reference's SoA device order
+ *     itself (tensorizeNetlist, tensors/physics_tensors.cpp:305-493);
+ *   - caller-owned host buffers, library-owned device buffers, opaque
+ *     handles; a batch is not re-entrant, different batches may be driven
+ *     from different host threads.
+ */
+#ifndef KARAM_NR_H
+#define KARAM_NR_H
+
+#include <stdint.h>
+
+#if defined(KARAM_NR_BUILD)
+#define NR_API __attribute__((visibility("default")))
+#else
+#define NR_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mirror SolverResult / SolverStats semantics) -------- */
+enum {
+    NR_OK = 0,
+    NR_ERR_INVALID = 1,      /* bad argument / shape                         */
+    NR_ERR_HIP = 2,          /* HIP runtime failure (message in nr_last_error)*/
+    NR_ERR_NO_DEVICE = 3,    /* no GPU visible                               */
+    NR_ERR_TOPOLOGY = 4,     /* netlist cannot be compiled (e.g. node gaps)  */
+    NR_ERR_OOM = 5,          /* device allocation failed                     */
+    NR_ERR_UNSUPPORTED = 6   /* feature not provided by this build           */
+};
+
+/* ---- netlist (caller-owned, netlist order) ------------------------------- */
+
+/* A `.model` card (netlist/circuit.h:31-41).  `type` is "NMOS", "PMOS" or
+ * "D"; has_* flags say whether the card sets the parameter (ModelCard::get
+ * falls back to the tensorizer defaults otherwise). */
+typedef struct nr_model_card {
+    const char* name;
+    const char* type;
+    int32_t has_kp, has_vto, has_lambda;
+    double kp, vto, lambda;
+} nr_model_card;
+
+typedef struct nr_netlist {
+    int32_t num_nodes;          /* numGlobalNodes                            */
+    double ambient_temp_K;      /* SimulationEnvironment, default 300.15     */
+
+    int32_t n_res;              /* Resistor (circuit.h)                      */
+    const int32_t* res_n1;
+    const int32_t* res_n2;
+    const double* res_ohms;
+
+    int32_t n_cap;              /* Capacitor                                 */
+    const int32_t* cap_n1;
+    const int32_t* cap_n2;
+    const double* cap_farads;
+
+    int32_t n_vsrc;             /* VoltageSource (circuit.h:143-171)         */
+    const int32_t* vsrc_pos;
+    const int32_t* vsrc_neg;
+    const double* vsrc_volts;   /* DC value (voltage_V)                      */
+    const int32_t* vsrc_is_pulse;  /* nullable: 1 = PULSE waveform           */
+    const double* vsrc_pulse;   /* nullable: 7 per source v1 v2 td tr tf pw per */
+
+    int32_t n_isrc;             /* CurrentSource                             */
+    const int32_t* isrc_pos;
+    const int32_t* isrc_neg;
+    const double* isrc_amps;
+
+    int32_t n_diode;            /* Diode (circuit.h:392-408)                 */
+    const int32_t* dio_anode;
+    const int32_t* dio_cathode;
+    const double* dio_is;
+    const double* dio_n;
+    const double* dio_vt;       /* thermalVoltage_V_T_V (unscaled)           */
+    const int32_t* dio_model;   /* nullable: index into `models` for the model
+                                   name; -1 / NULL = "diode_generic"          */
+
+    int32_t n_mos;              /* Mosfet (circuit.h:467-505)                */
+    const int32_t* mos_d;
+    const int32_t* mos_g;
+    const int32_t* mos_s;
+    const int32_t* mos_b;
+    const double* mos_w;
+    const double* mos_l;
+    const int32_t* mos_model;   /* index into `models` (model name + card)   */
+    const double* mos_kp;       /* nullable per-device KP override (MC)      */
+    const double* mos_vto;      /* nullable per-device VTO override (MC)     */
+
+    int32_t n_models;
+    const nr_model_card* models; /* model cards; an entry whose `type` is NULL
+                                    names a model with no card               */
+} nr_netlist;
+
+/* ---- per-instance parameter fields (Monte-Carlo / corners) -------------- */
+/* nr_batch_set_params(field, values[B * count]) — row-major per instance,
+ * netlist order, absolute values (already perturbed). */
+enum {
+    NR_FIELD_RES_OHMS = 0,   /* [B][n_res]   */
+    NR_FIELD_DIO_IS = 1,     /* [B][n_diode] */
+    NR_FIELD_MOS_KP = 2,     /* [B][n_mos]   */
+    NR_FIELD_MOS_VTO = 3,    /* [B][n_mos]   */
+    NR_FIELD_ISRC_AMPS = 4,  /* [B][n_isrc]  */
+    NR_FIELD_VSRC_VOLTS = 5, /* [B][n_vsrc]  */
+    NR_FIELD_CAP_FARADS = 6  /* [B][n_cap]   */
+};
+
+/* ---- opaque handles ------------------------------------------------------ */
+typedef struct nr_ctx nr_ctx;
+typedef struct nr_topology nr_topology;
+typedef struct nr_batch nr_batch;
+
+typedef struct nr_device_cfg {
+    int32_t device;          /* HIP device ordinal                           */
+    int32_t verbose;         /* 0 = silent                                   */
+} nr_device_cfg;
+
+/* ---- context ------------------------------------------------------------- */
+/* Replaces the GPU context manager singleton (gpu_context_manager.h:48-101)
+ * and the engine factory ISimulationEngine::create (engine_impl.cpp:339-341). */
+NR_API int nr_ctx_create(const nr_device_cfg* cfg, nr_ctx** out);
+NR_API int nr_ctx_destroy(nr_ctx* ctx);
+/* Last error message of this context (or of the library when ctx is NULL). */
+NR_API const char* nr_last_error(const nr_ctx* ctx);
+NR_API const char* nr_version(void);
+
+/* ---- topology (one-time symbolic analysis, host side) ------------------- */
+/* Tensorize (tensors/physics_tensors.cpp:305-493), CSR pattern + stamp map
+ * (math/linalg.h:281-465 MatrixConstructor::createCsr/buildPattern), factor
+ * graph + Min-Fill (infrastructure/factor_graph.cpp:9-162,
+ * infrastructure/treewidth_analyzer.cpp:14-141), symbolic factorisation
+ * (math/sparse_lu.cpp:14-128), supernodes and kernel schedules.  `ctx` may
+ * be NULL (pure host analysis). */
+NR_API int nr_topology_build(nr_ctx* ctx, const nr_netlist* nl, nr_topology** out);
+NR_API int nr_topology_destroy(nr_topology* topo);
+
+typedef struct nr_topology_info {
+    int32_t n;               /* matrix order (num_nodes)                     */
+    int32_t nnz_a;           /* CSR nonzeros of the stamped Jacobian         */
+    int32_t nnz_f;           /* nonzeros of the filled L+U pattern           */
+    int32_t n_stamps;        /* triplets per iteration (incl. gmin diagonal) */
+    int32_t treewidth;       /* Min-Fill estimatedTreewidth                  */
+    int32_t ordering_len;    /* raw Min-Fill ordering length (with branch ids)*/
+    int32_t fill_in;         /* SymbolicFactorization::estimatedFillIn       */
+    int32_t n_supernodes;
+    int32_t n_levels;        /* supernodal elimination-tree levels           */
+    int32_t max_front;       /* largest frontal matrix order                 */
+    int32_t n_res, n_vsrc, n_isrc, n_cap, n_diode, n_mos;
+    int64_t lu_flops;        /* multiply-add count x2 of one factorisation   */
+    int64_t u_entries;       /* stored factor entries per instance           */
+    double analyse_ms;       /* host time of nr_topology_build               */
+} nr_topology_info;
+NR_API int nr_topology_get_info(const nr_topology* topo, nr_topology_info* info);
+
+/* Integer structures for bit-exact comparison with the reference. */
+enum {
+    NR_EXPORT_ROW_PTR = 0,       /* int32[n+1]   CSR pattern of A            */
+    NR_EXPORT_COL_IDX = 1,       /* int32[nnz_a]                             */
+    NR_EXPORT_STAMP_SLOT = 2,    /* int32[n_stamps] triplet -> CSR slot      */
+    NR_EXPORT_ORDERING_RAW = 3,  /* int32[ordering_len] Min-Fill ids         */
+    NR_EXPORT_PERM = 4,          /* int32[n] perm[new] = old (0-based rows)  */
+    NR_EXPORT_FILL_ROW_PTR = 5,  /* int32[n+1] filled L+U pattern            */
+    NR_EXPORT_FILL_COL_IDX = 6,  /* int32[nnz_f]                             */
+    NR_EXPORT_DIODE_ORDER = 7,   /* int32[n_diode] SoA pos -> netlist index  */
+    NR_EXPORT_MOS_ORDER = 8      /* int32[n_mos]  SoA pos -> netlist index   */
+};
+NR_API int nr_topology_export(const nr_topology* topo, int32_t what,
+                              int32_t* out, int64_t capacity, int64_t* written);
+
+/* ---- batches -------------------------------------------------------------- */
+/* B instances of one topology, each with its own parameters (MC/corners).
+ * Device buffers are allocated here once; solves allocate nothing. */
+NR_API int nr_batch_alloc(nr_ctx* ctx, const nr_topology* topo, int32_t B,
+                          nr_batch** out);
+NR_API int nr_batch_free(nr_batch* batch);
+NR_API int nr_batch_set_params(nr_batch* batch, int32_t field,
+                               const double* values, int64_t count);
+/* Seeded start: v0[B * n].  Instances whose seed row is entirely NaN keep the
+ * plain seedInitialGuess start (solvers/circuitsim_solver.cpp:32-116). */
+NR_API int nr_batch_set_seed(nr_batch* batch, const double* v0, int64_t count);
+NR_API int nr_batch_clear_seed(nr_batch* batch);
+
+enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
+
+typedef struct nr_solve_opts {
+    int32_t max_iter;        /* per GMIN step, reference MAX_NR_ITER = 50     */
+    int32_t gmin_steps;      /* reference GMIN_STEPS = 19                     */
+    int32_t precision;       /* NR_PREC_*                                     */
+    int32_t seeded_start_step; /* GMIN step where seeded instances start      */
+    int32_t record_timing;   /* 1: per-phase HIP event timing                 */
+    int32_t use_graph;       /* 1: replay each batched iteration as a hipGraph*/
+} nr_solve_opts;
+NR_API void nr_solve_opts_default(nr_solve_opts* opts);
+
+typedef struct nr_result {
+    int32_t B;
+    int32_t n_converged;
+    int32_t batched_iterations;  /* device-side NR sweeps over the batch      */
+    int32_t max_iterations;      /* max per-instance NR iterations           */
+    double mean_iterations;
+    double max_kcl;              /* max over converged instances             */
+    double solve_ms;             /* wall time of nr_batch_solve_dc           */
+    double device_evals;         /* diode+MOSFET evaluations performed       */
+} nr_result;
+
+/* DC operating point of every instance: solveDC semantics
+ * (solvers/circuitsim_solver.cpp:176-555) with GMIN stepping, pnjlim,
+ * damping and the KCL test, batched on the GPU. */
+NR_API int nr_batch_solve_dc(nr_batch* batch, const nr_solve_opts* opts,
+                             nr_result* result);
+
+/* Per-instance results.  out arrays are caller-owned host buffers. */
+NR_API int nr_batch_get_voltages(const nr_batch* batch, double* out, int64_t count);
+enum {
+    NR_STATUS_CONVERGED = 0,
+    NR_STATUS_NOT_CONVERGED = 1,
+    NR_STATUS_SINGULAR = 2
+};
+typedef struct nr_instance_stats {
+    int32_t iterations;          /* true NR iteration count                  */
+    int32_t ref_iterations;      /* reference SolverStats accounting (D15)   */
+    int32_t status;              /* NR_STATUS_*                              */
+    int32_t gmin_steps_done;
+    double kcl;                  /* final max |KCL| (A), FP64                */
+} nr_instance_stats;
+NR_API int nr_batch_get_stats(const nr_batch* batch, nr_instance_stats* out,
+                              int64_t count);
+
+/* Per-phase device time of the last solve (milliseconds, summed over
+ * launches) when record_timing was set. */
+enum {
+    NR_PHASE_EVAL_STAMP = 0,
+    NR_PHASE_FACTOR = 1,
+    NR_PHASE_BACKSOLVE = 2,
+    NR_PHASE_UPDATE = 3,
+    NR_PHASE_COUNT = 4
+};
+typedef struct nr_timing {
+    double ms[NR_PHASE_COUNT];
+    int64_t launches[NR_PHASE_COUNT];
+    double factor_kernel_ms;     /* factor kernels only                       */
+    int64_t factor_instances;    /* instance-factorisations performed         */
+} nr_timing;
+NR_API int nr_batch_get_timing(const nr_batch* batch, nr_timing* out);
+
+/* ---- KernelDispatcher slots (tensors/kernel_dispatch.h:24-31) ----------- */
+/* Drop-in for batchDiodePhysics_generic (tensors/physics_tensors.cpp:19-51):
+ * host arrays in, host arrays out, evaluated on the GPU in FP64. */
+NR_API int nr_eval_diodes(nr_ctx* ctx, int32_t count,
+                          const int32_t* node_a, const int32_t* node_c,
+                          const double* Is, const double* N, const double* Vt,
+                          const double* voltages, int32_t n_voltages,
+                          double* v_d, double* i_d, double* g_d);
+/* Drop-in for batchMosfetPhysics_generic (tensors/physics_tensors.cpp:53-104). */
+NR_API int nr_eval_mosfets(nr_ctx* ctx, int32_t count,
+                           const int32_t* drains, const int32_t* gates,
+                           const int32_t* sources,
+                           const double* W, const double* L, const double* Kp,
+                           const double* Vth, const double* lambda,
+                           const int32_t* isPMOS,
+                           const double* voltages, int32_t n_voltages,
+                           double* vgs, double* vds, double* ids,
+                           double* gm, double* gds);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* KARAM_NR_H */

@@ -1,0 +1,256 @@
+"""Synthetic netlists of the benchmark families, built programmatically.
+
+Netlists are built the way the reference's own API builds them
+(TensorBlock::addDiode / addMosfet, netlist/circuit.h:815-843), never through
+the SPICE parser, whose diode and PULSE fields are left uninitialised.
+Node ids are 1-based, 0 is ground.  Every generator is seeded (numpy PCG64)
+so the oracle and the GPU library see identical inputs.
+
+Families:
+  inverter_chain  16-stage CMOS inverter chain (Level-1 NMOS/PMOS, 10 fF loads)
+  rd_mesh         side x side resistor mesh, a diode to ground at every 4th
+                  node, 8 current sources (the 64x64 Monte-Carlo workload)
+  random_logic    levelised random NAND2/INV netlist with local fan-in
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+
+VT_DEFAULT = 0.02585          # addDiode default thermal voltage (circuit.h:815)
+AMBIENT_K = 300.15            # SimulationEnvironment default
+
+
+def _i32(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.int32))
+
+
+def _f64(x):
+    return np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+
+
+@dataclass
+class ModelCard:
+    name: str
+    type: str | None            # "NMOS" / "PMOS" / "D"; None = name without a card
+    kp: float | None = None
+    vto: float | None = None
+    lam: float | None = None
+
+
+@dataclass
+class Netlist:
+    num_nodes: int
+    ambient_temp_K: float = AMBIENT_K
+    res_n1: np.ndarray = field(default_factory=lambda: _i32([]))
+    res_n2: np.ndarray = field(default_factory=lambda: _i32([]))
+    res_ohms: np.ndarray = field(default_factory=lambda: _f64([]))
+    cap_n1: np.ndarray = field(default_factory=lambda: _i32([]))
+    cap_n2: np.ndarray = field(default_factory=lambda: _i32([]))
+    cap_farads: np.ndarray = field(default_factory=lambda: _f64([]))
+    vsrc_pos: np.ndarray = field(default_factory=lambda: _i32([]))
+    vsrc_neg: np.ndarray = field(default_factory=lambda: _i32([]))
+    vsrc_volts: np.ndarray = field(default_factory=lambda: _f64([]))
+    vsrc_is_pulse: np.ndarray = field(default_factory=lambda: _i32([]))
+    vsrc_pulse: np.ndarray = field(default_factory=lambda: _f64(np.zeros((0, 7))))
+    isrc_pos: np.ndarray = field(default_factory=lambda: _i32([]))
+    isrc_neg: np.ndarray = field(default_factory=lambda: _i32([]))
+    isrc_amps: np.ndarray = field(default_factory=lambda: _f64([]))
+    dio_anode: np.ndarray = field(default_factory=lambda: _i32([]))
+    dio_cathode: np.ndarray = field(default_factory=lambda: _i32([]))
+    dio_is: np.ndarray = field(default_factory=lambda: _f64([]))
+    dio_n: np.ndarray = field(default_factory=lambda: _f64([]))
+    dio_vt: np.ndarray = field(default_factory=lambda: _f64([]))
+    dio_model: np.ndarray | None = None
+    mos_d: np.ndarray = field(default_factory=lambda: _i32([]))
+    mos_g: np.ndarray = field(default_factory=lambda: _i32([]))
+    mos_s: np.ndarray = field(default_factory=lambda: _i32([]))
+    mos_b: np.ndarray = field(default_factory=lambda: _i32([]))
+    mos_w: np.ndarray = field(default_factory=lambda: _f64([]))
+    mos_l: np.ndarray = field(default_factory=lambda: _f64([]))
+    mos_model: np.ndarray = field(default_factory=lambda: _i32([]))
+    mos_kp: np.ndarray | None = None
+    mos_vto: np.ndarray | None = None
+    models: list = field(default_factory=list)
+    name: str = "netlist"
+
+    # -- counts --------------------------------------------------------------
+    def field_count(self, fid: int) -> int:
+        return {
+            _abi.NR_FIELD_RES_OHMS: len(self.res_ohms),
+            _abi.NR_FIELD_DIO_IS: len(self.dio_is),
+            _abi.NR_FIELD_MOS_KP: len(self.mos_w),
+            _abi.NR_FIELD_MOS_VTO: len(self.mos_w),
+            _abi.NR_FIELD_ISRC_AMPS: len(self.isrc_amps),
+            _abi.NR_FIELD_VSRC_VOLTS: len(self.vsrc_volts),
+            _abi.NR_FIELD_CAP_FARADS: len(self.cap_farads),
+        }[fid]
+
+    def card_params(self):
+        """Per-device (KP, VTO) that the tensorizer derives from the cards
+        (tensors/physics_tensors.cpp:407-420), in netlist order."""
+        m = len(self.mos_w)
+        kp = np.full(m, 200e-6)
+        vto = np.full(m, 0.7)
+        any_card = any(c.type for c in self.models)
+        for i in range(m):
+            card = self.models[self.mos_model[i]] if self.mos_model[i] >= 0 else None
+            name = card.name if card else ""
+            if not any_card:
+                continue
+            real = next((c for c in self.models if c.type and c.name == name), None)
+            if real is not None:
+                pm = real.type == "PMOS"
+                kp[i] = real.kp if real.kp is not None else (100e-6 if pm else 200e-6)
+                vto[i] = real.vto if real.vto is not None else (-0.7 if pm else 0.7)
+            elif "PMOS" in name or "pmos" in name:
+                kp[i], vto[i] = 100e-6, -0.7
+        if self.mos_kp is not None:
+            kp = np.array(self.mos_kp, dtype=np.float64)
+        if self.mos_vto is not None:
+            vto = np.array(self.mos_vto, dtype=np.float64)
+        return kp, vto
+
+    # -- C view --------------------------------------------------------------
+    def to_c(self):
+        """Return (NrNetlist, keepalive).  keepalive must outlive the struct."""
+        keep = []
+
+        def p32(a):
+            if a is None:
+                return None
+            a = _i32(a)
+            keep.append(a)
+            return a.ctypes.data_as(_abi._i32p)
+
+        def p64(a):
+            if a is None:
+                return None
+            a = _f64(a)
+            keep.append(a)
+            return a.ctypes.data_as(_abi._f64p)
+
+        cards = (_abi.NrModelCard * max(1, len(self.models)))()
+        for i, m in enumerate(self.models):
+            nb = m.name.encode()
+            tb = m.type.encode() if m.type else None
+            keep.extend([nb, tb])
+            cards[i] = _abi.NrModelCard(nb, tb, int(m.kp is not None), int(m.vto is not None),
+                                        int(m.lam is not None), m.kp or 0.0, m.vto or 0.0,
+                                        m.lam or 0.0)
+        keep.append(cards)
+        nl = _abi.NrNetlist()
+        nl.num_nodes = int(self.num_nodes)
+        nl.ambient_temp_K = float(self.ambient_temp_K)
+        nl.n_res = len(self.res_ohms)
+        nl.res_n1, nl.res_n2, nl.res_ohms = p32(self.res_n1), p32(self.res_n2), p64(self.res_ohms)
+        nl.n_cap = len(self.cap_farads)
+        nl.cap_n1, nl.cap_n2, nl.cap_farads = p32(self.cap_n1), p32(self.cap_n2), p64(self.cap_farads)
+        nl.n_vsrc = len(self.vsrc_volts)
+        nl.vsrc_pos, nl.vsrc_neg, nl.vsrc_volts = p32(self.vsrc_pos), p32(self.vsrc_neg), p64(self.vsrc_volts)
+        if len(self.vsrc_is_pulse) == nl.n_vsrc and nl.n_vsrc:
+            nl.vsrc_is_pulse = p32(self.vsrc_is_pulse)
+            nl.vsrc_pulse = p64(np.asarray(self.vsrc_pulse).reshape(-1))
+        nl.n_isrc = len(self.isrc_amps)
+        nl.isrc_pos, nl.isrc_neg, nl.isrc_amps = p32(self.isrc_pos), p32(self.isrc_neg), p64(self.isrc_amps)
+        nl.n_diode = len(self.dio_is)
+        nl.dio_anode, nl.dio_cathode = p32(self.dio_anode), p32(self.dio_cathode)
+        nl.dio_is, nl.dio_n, nl.dio_vt = p64(self.dio_is), p64(self.dio_n), p64(self.dio_vt)
+        nl.dio_model = p32(self.dio_model)
+        nl.n_mos = len(self.mos_w)
+        nl.mos_d, nl.mos_g, nl.mos_s, nl.mos_b = (p32(self.mos_d), p32(self.mos_g),
+                                                  p32(self.mos_s), p32(self.mos_b))
+        nl.mos_w, nl.mos_l, nl.mos_model = p64(self.mos_w), p64(self.mos_l), p32(self.mos_model)
+        nl.mos_kp, nl.mos_vto = p64(self.mos_kp), p64(self.mos_vto)
+        nl.n_models = len(self.models)
+        nl.models = C.cast(cards, C.POINTER(_abi.NrModelCard))
+        return nl, keep
+
+
+# ---------------------------------------------------------------------------
+# families
+# ---------------------------------------------------------------------------
+NMOS_CARD = ModelCard("NMOS", "NMOS", kp=110e-6, vto=0.7, lam=0.04)
+PMOS_CARD = ModelCard("PMOS", "PMOS", kp=50e-6, vto=-0.7, lam=0.05)
+
+
+def inverter_chain(stages: int = 16, vdd: float = 1.8, cload: float = 10e-15,
+                   pulse=(0.0, 1.8, 20e-9, 1e-9, 1e-9, 1.0, 0.0)) -> Netlist:
+    """CMOS inverter chain: VDD = node 1, IN = node 2, OUT_k = node 2 + k.
+    NMOS W/L = 2/1, PMOS W/L = 4/1, a load capacitor per output; V_IN is a
+    PULSE source whose DC value is its v1 (D8)."""
+    n = 2 + stages
+    md, mg, ms, mb, mw, ml, mm = [], [], [], [], [], [], []
+    for k in range(1, stages + 1):
+        out = 2 + k
+        gate = 2 if k == 1 else out - 1
+        md += [out, out]; mg += [gate, gate]; ms += [0, 1]; mb += [0, 1]
+        mw += [2e-6, 4e-6]; ml += [1e-6, 1e-6]; mm += [0, 1]
+    return Netlist(
+        num_nodes=n,
+        cap_n1=_i32(range(3, n + 1)), cap_n2=_i32([0] * stages), cap_farads=_f64([cload] * stages),
+        vsrc_pos=_i32([1, 2]), vsrc_neg=_i32([0, 0]), vsrc_volts=_f64([vdd, pulse[0]]),
+        vsrc_is_pulse=_i32([0, 1]), vsrc_pulse=_f64([[0] * 7, list(pulse)]),
+        mos_d=_i32(md), mos_g=_i32(mg), mos_s=_i32(ms), mos_b=_i32(mb),
+        mos_w=_f64(mw), mos_l=_f64(ml), mos_model=_i32(mm),
+        models=[NMOS_CARD, PMOS_CARD], name=f"inv{stages}",
+    )
+
+
+def rd_mesh(side: int = 64, seed: int = 1, n_isrc: int = 8,
+            r_lo: float = 1e3, r_hi: float = 10e3, i_lo: float = 1e-3, i_hi: float = 5e-3,
+            diode_every: int = 4, Is: float = 1e-14) -> Netlist:
+    """side x side resistor mesh, node id = 1 + side*r + c.  Resistors: all
+    horizontal edges row-major, then all vertical edges; R ~ U(r_lo, r_hi).
+    A diode (anode = node, cathode = ground) at every node with
+    (id - 1) % diode_every == 0; n_isrc current sources (pos = ground,
+    neg = node) inject U(i_lo, i_hi) into distinct seeded nodes."""
+    rng = np.random.default_rng(seed)
+    n = side * side
+    ids = np.arange(1, n + 1).reshape(side, side)
+    h1, h2 = ids[:, :-1].ravel(), ids[:, 1:].ravel()
+    v1, v2 = ids[:-1, :].ravel(), ids[1:, :].ravel()
+    n1 = np.concatenate([h1, v1]); n2 = np.concatenate([h2, v2])
+    R = rng.uniform(r_lo, r_hi, size=n1.size)
+    an = np.arange(1, n + 1)[(np.arange(1, n + 1) - 1) % diode_every == 0]
+    isn = rng.choice(np.arange(1, n + 1), size=min(n_isrc, n), replace=False)
+    I = rng.uniform(i_lo, i_hi, size=isn.size)
+    return Netlist(
+        num_nodes=n, res_n1=_i32(n1), res_n2=_i32(n2), res_ohms=_f64(R),
+        isrc_pos=_i32(np.zeros(isn.size)), isrc_neg=_i32(isn), isrc_amps=_f64(I),
+        dio_anode=_i32(an), dio_cathode=_i32(np.zeros(an.size)), dio_is=_f64(np.full(an.size, Is)),
+        dio_n=_f64(np.ones(an.size)), dio_vt=_f64(np.full(an.size, VT_DEFAULT)),
+        name=f"rdmesh{side}",
+    )
+
+
+def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: float = 0.05,
+              sigma_vto: float = 0.0, sigma_kp: float = 0.0) -> dict:
+    """Per-instance Monte-Carlo parameter arrays [B, count] (netlist order):
+    R and IS multiplied by max(0.05, 1 + sigma z); VTO += sigma_vto z,
+    KP *= max(0.05, 1 + sigma_kp z); z ~ N(0, 1), instance b drawn from
+    seed + b so any instance can be regenerated on its own."""
+    out = {}
+    nr, nd, nm = len(nl.res_ohms), len(nl.dio_is), len(nl.mos_w)
+    R = np.empty((B, nr)); Isa = np.empty((B, nd))
+    kp0, vto0 = nl.card_params()
+    KP = np.empty((B, nm)); VT = np.empty((B, nm))
+    for b in range(B):
+        rng = np.random.default_rng(seed + b)
+        R[b] = nl.res_ohms * np.maximum(0.05, 1.0 + sigma_r * rng.standard_normal(nr))
+        Isa[b] = nl.dio_is * np.maximum(0.05, 1.0 + sigma_is * rng.standard_normal(nd))
+        if nm:
+            VT[b] = vto0 + sigma_vto * rng.standard_normal(nm)
+            KP[b] = kp0 * np.maximum(0.05, 1.0 + sigma_kp * rng.standard_normal(nm))
+    if nr:
+        out[_abi.NR_FIELD_RES_OHMS] = R
+    if nd:
+        out[_abi.NR_FIELD_DIO_IS] = Isa
+    if nm and (sigma_vto or sigma_kp):
+        out[_abi.NR_FIELD_MOS_KP] = KP
+        out[_abi.NR_FIELD_MOS_VTO] = VT
+    return out

@@ -1,0 +1,233 @@
+"""Python binding of the C ABI in include/karam_nr.h (libkaram_nr.so).
+
+Mirrors the reference's operator surface for the NR hot path:
+  - `Context`      GPU context (replaces the GPU context manager singleton)
+  - `Topology`     one-time symbolic analysis (NetlistCompiler::compile +
+                   symbolicFactorize, infrastructure/netlist_compiler.cpp:17-89)
+  - `Batch`        B Monte-Carlo / corner instances of one topology, solved
+                   together by CircuitSim::solveDC semantics
+                   (solvers/circuitsim_solver.cpp:176-555)
+  - `eval_diodes` / `eval_mosfets`   KernelDispatcher slot replacements
+                   (tensors/kernel_dispatch.h:24-31)
+
+The HIP library is the only compute path: if libkaram_nr.so is missing or no
+GPU is visible, calls raise instead of falling back to anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libkaram_nr.so")
+_LIB = None
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+# every symbol the C header declares (checked by the CPU test suite)
+EXPORTED = [
+    "nr_ctx_create", "nr_ctx_destroy", "nr_last_error", "nr_version",
+    "nr_topology_build", "nr_topology_destroy", "nr_topology_get_info", "nr_topology_export",
+    "nr_batch_alloc", "nr_batch_free", "nr_batch_set_params", "nr_batch_set_seed",
+    "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
+    "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing",
+    "nr_eval_diodes", "nr_eval_mosfets",
+]
+
+
+class NrError(RuntimeError):
+    pass
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise NrError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.nr_version.restype = C.c_char_p
+        L.nr_last_error.restype = C.c_char_p
+        L.nr_last_error.argtypes = [_vp]
+        L.nr_ctx_create.argtypes = [C.POINTER(_abi.NrDeviceCfg), C.POINTER(_vp)]
+        L.nr_ctx_destroy.argtypes = [_vp]
+        L.nr_topology_build.argtypes = [_vp, C.POINTER(_abi.NrNetlist), C.POINTER(_vp)]
+        L.nr_topology_destroy.argtypes = [_vp]
+        L.nr_topology_get_info.argtypes = [_vp, C.POINTER(_abi.NrTopologyInfo)]
+        L.nr_topology_export.argtypes = [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]
+        L.nr_batch_alloc.argtypes = [_vp, _vp, C.c_int32, C.POINTER(_vp)]
+        L.nr_batch_free.argtypes = [_vp]
+        L.nr_batch_set_params.argtypes = [_vp, C.c_int32, _f64p, C.c_int64]
+        L.nr_batch_set_seed.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_clear_seed.argtypes = [_vp]
+        L.nr_solve_opts_default.argtypes = [C.POINTER(_abi.NrSolveOpts)]
+        L.nr_solve_opts_default.restype = None
+        L.nr_batch_solve_dc.argtypes = [_vp, C.POINTER(_abi.NrSolveOpts), C.POINTER(_abi.NrResult)]
+        L.nr_batch_get_voltages.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_stats.argtypes = [_vp, C.POINTER(_abi.NrInstanceStats), C.c_int64]
+        L.nr_batch_get_timing.argtypes = [_vp, C.POINTER(_abi.NrTiming)]
+        L.nr_eval_diodes.argtypes = [_vp, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int32,
+                                     _f64p, _f64p, _f64p]
+        L.nr_eval_mosfets.argtypes = [_vp, C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                      _i32p, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p]
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, ctx=None):
+    if rc != _abi.NR_OK:
+        msg = lib().nr_last_error(ctx.h if ctx is not None else None)
+        raise NrError(f"karam_nr error {rc}: {msg.decode() if msg else ''}")
+
+
+def _p64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_i32p)
+
+
+class Context:
+    def __init__(self, device: int = 0, verbose: int = 0):
+        h = _vp()
+        cfg = _abi.NrDeviceCfg(device, verbose)
+        rc = lib().nr_ctx_create(C.byref(cfg), C.byref(h))
+        _check(rc)
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nr_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Topology:
+    """Symbolic analysis of a netlist (host side; needs no GPU)."""
+
+    def __init__(self, nl, ctx: Context | None = None):
+        self.nl = nl
+        self._c, self._keep = nl.to_c()
+        h = _vp()
+        rc = lib().nr_topology_build(ctx.h if ctx else None, C.byref(self._c), C.byref(h))
+        _check(rc, ctx)
+        self.h = h
+        info = _abi.NrTopologyInfo()
+        _check(lib().nr_topology_get_info(h, C.byref(info)))
+        self.info = {k: getattr(info, k) for k, _ in _abi.NrTopologyInfo._fields_}
+
+    def export(self, what):
+        n = C.c_int64()
+        _check(lib().nr_topology_export(self.h, what, None, 0, C.byref(n)))
+        buf = np.zeros(max(1, n.value), dtype=np.int32)
+        _check(lib().nr_topology_export(self.h, what, _p32(buf), buf.size, C.byref(n)))
+        return buf[: n.value].copy()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nr_topology_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def solve_opts(max_iter=50, gmin_steps=19, precision=_abi.NR_PREC_FP64, seeded_start_step=18,
+               record_timing=0, use_graph=0):
+    o = _abi.NrSolveOpts()
+    lib().nr_solve_opts_default(C.byref(o))
+    o.max_iter, o.gmin_steps, o.precision = max_iter, gmin_steps, precision
+    o.seeded_start_step, o.record_timing, o.use_graph = seeded_start_step, record_timing, use_graph
+    return o
+
+
+class Batch:
+    """B instances of one topology; parameters per instance (MC / corners)."""
+
+    def __init__(self, ctx: Context, topo: Topology, B: int):
+        self.ctx, self.topo, self.B = ctx, topo, int(B)
+        h = _vp()
+        _check(lib().nr_batch_alloc(ctx.h, topo.h, self.B, C.byref(h)), ctx)
+        self.h = h
+
+    def set_params(self, fid: int, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        _check(lib().nr_batch_set_params(self.h, fid, _p64(a), a.size), self.ctx)
+
+    def set_fields(self, fields: dict):
+        for fid, arr in fields.items():
+            self.set_params(fid, np.asarray(arr)[: self.B])
+
+    def set_seed(self, v0):
+        a = np.ascontiguousarray(v0, dtype=np.float64)
+        _check(lib().nr_batch_set_seed(self.h, _p64(a), a.size), self.ctx)
+
+    def clear_seed(self):
+        _check(lib().nr_batch_clear_seed(self.h), self.ctx)
+
+    def solve_dc(self, opts=None, **kw):
+        o = opts if opts is not None else solve_opts(**kw)
+        r = _abi.NrResult()
+        _check(lib().nr_batch_solve_dc(self.h, C.byref(o), C.byref(r)), self.ctx)
+        return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def voltages(self):
+        n = self.topo.info["n"]
+        out = np.zeros((self.B, n))
+        _check(lib().nr_batch_get_voltages(self.h, _p64(out), out.size), self.ctx)
+        return out
+
+    def stats(self):
+        arr = (_abi.NrInstanceStats * self.B)()
+        _check(lib().nr_batch_get_stats(self.h, arr, self.B), self.ctx)
+        return {
+            "iterations": np.array([x.iterations for x in arr]),
+            "ref_iterations": np.array([x.ref_iterations for x in arr]),
+            "status": np.array([x.status for x in arr]),
+            "gmin_steps_done": np.array([x.gmin_steps_done for x in arr]),
+            "kcl": np.array([x.kcl for x in arr]),
+        }
+
+    def timing(self):
+        t = _abi.NrTiming()
+        _check(lib().nr_batch_get_timing(self.h, C.byref(t)), self.ctx)
+        return {"ms": list(t.ms), "launches": list(t.launches), "factor_kernel_ms": t.factor_kernel_ms,
+                "factor_instances": t.factor_instances}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nr_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def eval_diodes(ctx: Context, a, c, Is, N, Vt, v):
+    """KernelDispatcher::batch_diode_physics replacement (host arrays)."""
+    a, c = np.ascontiguousarray(a, np.int32), np.ascontiguousarray(c, np.int32)
+    Is, N, Vt, v = (np.ascontiguousarray(x, np.float64) for x in (Is, N, Vt, v))
+    m = len(a)
+    vd, i_d, gd = np.zeros(m), np.zeros(m), np.zeros(m)
+    _check(lib().nr_eval_diodes(ctx.h, m, _p32(a), _p32(c), _p64(Is), _p64(N), _p64(Vt), _p64(v), len(v),
+                                _p64(vd), _p64(i_d), _p64(gd)), ctx)
+    return vd, i_d, gd
+
+
+def eval_mosfets(ctx: Context, d, g, s, W, L, Kp, Vth, lam, pmos, v):
+    """KernelDispatcher::batch_mosfet_physics replacement (host arrays)."""
+    d, g, s, pmos = (np.ascontiguousarray(x, np.int32) for x in (d, g, s, pmos))
+    W, L, Kp, Vth, lam, v = (np.ascontiguousarray(x, np.float64) for x in (W, L, Kp, Vth, lam, v))
+    m = len(d)
+    outs = [np.zeros(m) for _ in range(5)]
+    _check(lib().nr_eval_mosfets(ctx.h, m, _p32(d), _p32(g), _p32(s), _p64(W), _p64(L), _p64(Kp), _p64(Vth),
+                                 _p64(lam), _p32(pmos), _p64(v), len(v), *[_p64(o) for o in outs]), ctx)
+    return tuple(outs)

@@ -1,0 +1,1101 @@
+/*
+ * oracle.c — CPU FP64 restatement of the reference NR DC path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Built without fast-math.  Each
+ * function follows the reference file:line named in its comment; deviations
+ * from literal reference behaviour carry the defect id of the hot-path
+ * survey's register (D3, D4, D7, D14, D15).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define GMIN_DEV 1e-12
+
+/* ------------------------------------------------------------------------ */
+/* small helpers                                                            */
+/* ------------------------------------------------------------------------ */
+static void* xcalloc(size_t n, size_t sz) {
+    void* p = calloc(n ? n : 1, sz);
+    if (!p) abort();
+    return p;
+}
+static void* xmalloc(size_t sz) {
+    void* p = malloc(sz ? sz : 1);
+    if (!p) abort();
+    return p;
+}
+static double dmin_ref(double a, double b) { return (b < a) ? b : a; }  /* std::min(a,b) */
+static double dmax_ref(double a, double b) { return (a < b) ? b : a; }  /* std::max(a,b) */
+
+/* ------------------------------------------------------------------------ */
+/* a2: tensorizeNetlist (tensors/physics_tensors.cpp:305-493)               */
+/* ------------------------------------------------------------------------ */
+struct orc_soa {
+    int32_t n;
+    int32_t nD, nM;
+    int32_t *da, *dc, *d_src;
+    double *dIs, *dN, *dVt;
+    int32_t *md, *mg, *ms, *mb, *mP, *m_src;
+    double *mW, *mL, *mKp, *mVth, *mLam;
+};
+
+static const char* model_name(const nr_netlist* nl, int32_t idx, const char* dflt) {
+    if (idx < 0 || idx >= nl->n_models || !nl->models[idx].name) return dflt;
+    return nl->models[idx].name;
+}
+
+/* Find the card named `name` (first match), NULL if none. */
+static const nr_model_card* find_card(const nr_netlist* nl, const char* name) {
+    for (int32_t i = 0; i < nl->n_models; ++i) {
+        const nr_model_card* c = &nl->models[i];
+        if (c->type && c->name && strcmp(c->name, name) == 0) return c;
+    }
+    return NULL;
+}
+static int has_any_card(const nr_netlist* nl) {
+    for (int32_t i = 0; i < nl->n_models; ++i)
+        if (nl->models[i].type) return 1;
+    return 0;
+}
+
+/* stable sort of idx[] by name via insertion/merge (std::map bucket order:
+ * lexicographic model name, netlist order within a bucket). */
+typedef struct { const char* key; int32_t idx; } name_item;
+static void stable_sort_names(name_item* it, int32_t n) {
+    if (n < 2) return;
+    name_item* tmp = (name_item*)xmalloc(sizeof(name_item) * (size_t)n);
+    for (int32_t w = 1; w < n; w *= 2) {
+        for (int32_t lo = 0; lo < n; lo += 2 * w) {
+            int32_t mid = lo + w < n ? lo + w : n, hi = lo + 2 * w < n ? lo + 2 * w : n;
+            int32_t i = lo, j = mid, k = lo;
+            while (i < mid && j < hi) {
+                if (strcmp(it[j].key, it[i].key) < 0) tmp[k++] = it[j++];
+                else tmp[k++] = it[i++];
+            }
+            while (i < mid) tmp[k++] = it[i++];
+            while (j < hi) tmp[k++] = it[j++];
+        }
+        memcpy(it, tmp, sizeof(name_item) * (size_t)n);
+    }
+    free(tmp);
+}
+
+orc_soa* orc_tensorize(const nr_netlist* nl) {
+    orc_soa* s = (orc_soa*)xcalloc(1, sizeof(orc_soa));
+    s->n = nl->num_nodes;
+    const double tscale = nl->ambient_temp_K / 300.15;   /* physics_tensors.cpp:419 */
+    /* diodes: bucket by modelName (default "diode_generic", circuit.h:404) */
+    s->nD = nl->n_diode;
+    name_item* items = (name_item*)xmalloc(sizeof(name_item) * (size_t)(s->nD + 1));
+    for (int32_t i = 0; i < s->nD; ++i) {
+        items[i].key = model_name(nl, nl->dio_model ? nl->dio_model[i] : -1, "diode_generic");
+        items[i].idx = i;
+    }
+    stable_sort_names(items, s->nD);
+    s->da = (int32_t*)xcalloc((size_t)s->nD, 4); s->dc = (int32_t*)xcalloc((size_t)s->nD, 4);
+    s->d_src = (int32_t*)xcalloc((size_t)s->nD, 4);
+    s->dIs = (double*)xcalloc((size_t)s->nD, 8); s->dN = (double*)xcalloc((size_t)s->nD, 8);
+    s->dVt = (double*)xcalloc((size_t)s->nD, 8);
+    for (int32_t p = 0; p < s->nD; ++p) {
+        int32_t i = items[p].idx;
+        s->d_src[p] = i;
+        s->da[p] = nl->dio_anode[i]; s->dc[p] = nl->dio_cathode[i];
+        s->dIs[p] = nl->dio_is[i]; s->dN[p] = nl->dio_n[i];
+        s->dVt[p] = nl->dio_vt[i] * tscale;
+    }
+    free(items);
+    /* MOSFETs: bucket by modelName; parameters from the card (:424-441 via
+     * collectFromBlock :407-420); no cards at all -> NMOS defaults (D9). */
+    s->nM = nl->n_mos;
+    items = (name_item*)xmalloc(sizeof(name_item) * (size_t)(s->nM + 1));
+    for (int32_t i = 0; i < s->nM; ++i) {
+        items[i].key = model_name(nl, nl->mos_model ? nl->mos_model[i] : -1, "");
+        items[i].idx = i;
+    }
+    stable_sort_names(items, s->nM);
+    size_t m = (size_t)s->nM;
+    s->md = (int32_t*)xcalloc(m, 4); s->mg = (int32_t*)xcalloc(m, 4); s->ms = (int32_t*)xcalloc(m, 4);
+    s->mb = (int32_t*)xcalloc(m, 4); s->mP = (int32_t*)xcalloc(m, 4); s->m_src = (int32_t*)xcalloc(m, 4);
+    s->mW = (double*)xcalloc(m, 8); s->mL = (double*)xcalloc(m, 8); s->mKp = (double*)xcalloc(m, 8);
+    s->mVth = (double*)xcalloc(m, 8); s->mLam = (double*)xcalloc(m, 8);
+    const int cards = has_any_card(nl);
+    for (int32_t p = 0; p < s->nM; ++p) {
+        int32_t i = items[p].idx;
+        const char* name = items[p].key;
+        int pm = 0;
+        double kp = 200e-6, vth = 0.7, lam = 0.02;
+        if (cards) {
+            const nr_model_card* c = find_card(nl, name);
+            if (c) {
+                pm = strcmp(c->type, "PMOS") == 0;
+                kp = c->has_kp ? c->kp : (pm ? 100e-6 : 200e-6);
+                vth = c->has_vto ? c->vto : (pm ? -0.7 : 0.7);
+                lam = c->has_lambda ? c->lambda : 0.02;
+            } else {
+                pm = (strstr(name, "PMOS") != NULL) || (strstr(name, "pmos") != NULL);
+                if (pm) { kp = 100e-6; vth = -0.7; }
+            }
+        }
+        /* D14 extension: per-device Monte-Carlo KP / VTO overrides */
+        if (nl->mos_kp) kp = nl->mos_kp[i];
+        if (nl->mos_vto) vth = nl->mos_vto[i];
+        s->m_src[p] = i;
+        s->md[p] = nl->mos_d[i]; s->mg[p] = nl->mos_g[i]; s->ms[p] = nl->mos_s[i]; s->mb[p] = nl->mos_b[i];
+        s->mW[p] = nl->mos_w[i]; s->mL[p] = nl->mos_l[i];
+        s->mKp[p] = kp; s->mVth[p] = vth; s->mLam[p] = lam; s->mP[p] = pm;
+    }
+    free(items);
+    return s;
+}
+
+void orc_soa_free(orc_soa* s) {
+    if (!s) return;
+    free(s->da); free(s->dc); free(s->d_src); free(s->dIs); free(s->dN); free(s->dVt);
+    free(s->md); free(s->mg); free(s->ms); free(s->mb); free(s->mP); free(s->m_src);
+    free(s->mW); free(s->mL); free(s->mKp); free(s->mVth); free(s->mLam);
+    free(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4: batchDiodePhysics_generic (tensors/physics_tensors.cpp:19-51)        */
+/* ------------------------------------------------------------------------ */
+static inline double volt(const double* v, int32_t nv, int32_t node) {
+    return (node > 0 && node <= nv) ? v[node - 1] : 0.0;
+}
+
+void orc_eval_diodes(int32_t count, const int32_t* a, const int32_t* c,
+                     const double* Is, const double* N, const double* Vt,
+                     const double* v, int32_t nv,
+                     double* vd, double* id, double* gd) {
+    for (int32_t i = 0; i < count; ++i) {
+        double v_d = volt(v, nv, a[i]) - volt(v, nv, c[i]);
+        vd[i] = v_d;
+        double n_vt = N[i] * Vt[i];
+        double vcrit = 30.0 * n_vt;
+        double arg = v_d / n_vt;
+        double arg_c = dmin_ref(arg, 30.0);
+        double e = exp(arg_c);
+        double i_base = Is[i] * (e - 1.0);
+        double g_base = (Is[i] / n_vt) * e;
+        double delta = dmax_ref(0.0, v_d - vcrit);
+        id[i] = i_base + delta * g_base;
+        gd[i] = dmax_ref(GMIN_DEV, g_base);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a5: batchMosfetPhysics_generic (tensors/physics_tensors.cpp:53-104)      */
+/* ------------------------------------------------------------------------ */
+static inline void mos_one(double vg, double vd_, double vs, double W, double L,
+                           double Kp, double Vth, double lam, int pmos,
+                           double* o_vgs, double* o_vds, double* o_ids,
+                           double* o_gm, double* o_gds) {
+    double sign = pmos ? -1.0 : 1.0;
+    double vgs = vg - vs, vds = vd_ - vs;
+    double lvgs = sign * vgs, lvds = sign * vds, lvth = sign * Vth;
+    double K = 0.5 * Kp * (W / L);
+    double vov = lvgs - lvth;
+    double ids, gm, gds;
+    if (vov <= 0.0) {
+        ids = 0.0; gm = 0.0; gds = 1e-12;
+    } else if (lvds < vov) {
+        ids = sign * K * (2.0 * vov * lvds - lvds * lvds);
+        gm = 2.0 * K * lvds;
+        gds = 2.0 * K * (vov - lvds);
+    } else {
+        ids = sign * K * vov * vov * (1.0 + lam * lvds);
+        gm = 2.0 * K * vov * (1.0 + lam * lvds);
+        gds = K * vov * vov * lam;
+    }
+    if (gds < 1e-12) gds = 1e-12;
+    *o_vgs = vgs; *o_vds = vds; *o_ids = ids; *o_gm = gm; *o_gds = gds;
+}
+
+void orc_eval_mosfets(int32_t count, const int32_t* d, const int32_t* g,
+                      const int32_t* s, const double* W, const double* L,
+                      const double* Kp, const double* Vth, const double* lam,
+                      const int32_t* pmos, const double* v, int32_t nv,
+                      double* vgs, double* vds, double* ids, double* gm,
+                      double* gds) {
+    for (int32_t i = 0; i < count; ++i)
+        mos_one(volt(v, nv, g[i]), volt(v, nv, d[i]), volt(v, nv, s[i]), W[i], L[i],
+                Kp[i], Vth[i], lam[i], pmos[i], &vgs[i], &vds[i], &ids[i], &gm[i], &gds[i]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Topology: stamps, pattern, Min-Fill, symbolic                            */
+/* ------------------------------------------------------------------------ */
+struct orc_topo {
+    int32_t n;
+    orc_soa* soa;                  /* SoA of the template netlist */
+    int32_t n_stamps;
+    int32_t *st_row, *st_col;      /* triplet coordinates in emission order */
+    int32_t nnz_a;
+    int32_t *row_ptr, *col_idx;    /* CSR pattern (createCsr) */
+    int32_t* stamp_slot;
+    int32_t n_vars, ordering_len, treewidth;
+    int32_t* ordering_raw;         /* Min-Fill ids incl. branch ids */
+    int32_t *perm, *iperm;
+    int32_t nnz_f, fill_in;
+    int32_t *frow_ptr, *fcol_idx, *fdiag;
+};
+
+/* growable int vector */
+typedef struct { int32_t* a; int32_t n, cap; } ivec;
+static void iv_push(ivec* v, int32_t x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? v->cap * 2 : 8;
+        v->a = (int32_t*)realloc(v->a, sizeof(int32_t) * (size_t)v->cap);
+        if (!v->a) abort();
+    }
+    v->a[v->n++] = x;
+}
+
+/* Emit the DC triplet coordinates in stampSoABlock order
+ * (physics/circuitsim.h:2413-2751, h = 0) followed by the gmin diagonal
+ * (addDiagonalConditioning, :2857-2860).  When vals != NULL the values and
+ * the rhs are produced as well (one NR iteration's stamping). */
+typedef struct {
+    ivec rows, cols;
+    double* vals; int32_t nv;      /* optional values */
+    double* rhs;
+} emitter;
+static inline void emit(emitter* e, int32_t r, int32_t c, double val) {
+    if (e->vals) { e->vals[e->nv++] = val; }
+    else { iv_push(&e->rows, r); iv_push(&e->cols, c); }
+}
+
+static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
+                      double gmin, emitter* e) {
+    const int32_t n = nl->num_nodes;
+    double* rhs = e->rhs;
+    /* (1) resistors :2416-2438 */
+    for (int32_t i = 0; i < nl->n_res; ++i) {
+        int32_t n1 = nl->res_n1[i], n2 = nl->res_n2[i];
+        double g = 1.0 / nl->res_ohms[i];
+        if (n1 != 0 && n2 != 0) {
+            emit(e, n1 - 1, n1 - 1, g); emit(e, n2 - 1, n2 - 1, g);
+            emit(e, n1 - 1, n2 - 1, -g); emit(e, n2 - 1, n1 - 1, -g);
+        } else if (n1 != 0) emit(e, n1 - 1, n1 - 1, g);
+        else if (n2 != 0) emit(e, n2 - 1, n2 - 1, g);
+    }
+    /* (2) voltage sources, Norton 1e3 S :2440-2467 */
+    for (int32_t i = 0; i < nl->n_vsrc; ++i) {
+        int32_t p = nl->vsrc_pos[i], q = nl->vsrc_neg[i];
+        double V = nl->vsrc_volts[i];
+        double g = 1.0e3, ieq = V * g;
+        if (p != 0 && q != 0) {
+            emit(e, p - 1, p - 1, g); emit(e, q - 1, q - 1, g);
+            emit(e, p - 1, q - 1, -g); emit(e, q - 1, p - 1, -g);
+        } else if (p != 0) emit(e, p - 1, p - 1, g);
+        else if (q != 0) emit(e, q - 1, q - 1, g);
+        if (rhs) {
+            if (p != 0) rhs[p - 1] += ieq;
+            if (q != 0) rhs[q - 1] -= ieq;
+        }
+    }
+    /* (3) current sources :2469-2477 */
+    if (rhs) {
+        for (int32_t i = 0; i < nl->n_isrc; ++i) {
+            int32_t p = nl->isrc_pos[i], q = nl->isrc_neg[i];
+            double I = nl->isrc_amps[i];
+            if (p != 0) rhs[p - 1] -= I;
+            if (q != 0) rhs[q - 1] += I;
+        }
+    }
+    /* (6) device evaluation :2535-2537, (7) diodes :2543-2576 */
+    for (int32_t i = 0; i < s->nD; ++i) {
+        int32_t a = s->da[i], c = s->dc[i];
+        double g = 0.0, ieq = 0.0;
+        if (e->vals) {
+            double vd, id, gd;
+            orc_eval_diodes(1, &a, &c, &s->dIs[i], &s->dN[i], &s->dVt[i], v, n, &vd, &id, &gd);
+            g = gd; ieq = id - g * vd;
+        }
+        if (a != 0 && c != 0) {
+            emit(e, a - 1, a - 1, g); emit(e, c - 1, c - 1, g);
+            emit(e, a - 1, c - 1, -g); emit(e, c - 1, a - 1, -g);
+        } else if (a != 0) emit(e, a - 1, a - 1, g);
+        else if (c != 0) emit(e, c - 1, c - 1, g);
+        if (rhs) {
+            if (a != 0) rhs[a - 1] -= ieq;
+            if (c != 0) rhs[c - 1] += ieq;
+        }
+    }
+    /* (8) MOSFETs :2592-2657 (gmb = 0 from the generic kernel) */
+    for (int32_t i = 0; i < s->nM; ++i) {
+        int32_t d = s->md[i], g = s->mg[i], so = s->ms[i], b = s->mb[i];
+        double gm = 0, gmb = 0, gds = 0, ieq = 0;
+        if (e->vals) {
+            double vgs, vds, ids;
+            mos_one(volt(v, n, g), volt(v, n, d), volt(v, n, so), s->mW[i], s->mL[i],
+                    s->mKp[i], s->mVth[i], s->mLam[i], s->mP[i], &vgs, &vds, &ids, &gm, &gds);
+            double vbs = (b != 0) ? (v[b - 1] - (so != 0 ? v[so - 1] : 0.0)) : 0.0;
+            ieq = ids - (gm * vgs + gmb * vbs + gds * vds);
+        }
+        if (d != 0 && so != 0) {
+            emit(e, d - 1, d - 1, gds); emit(e, so - 1, so - 1, gds);
+            emit(e, d - 1, so - 1, -gds); emit(e, so - 1, d - 1, -gds);
+        } else if (d != 0) emit(e, d - 1, d - 1, gds);
+        else if (so != 0) emit(e, so - 1, so - 1, gds);
+        if (d != 0) {
+            if (g != 0) emit(e, d - 1, g - 1, gm);
+            if (so != 0) emit(e, d - 1, so - 1, -gm);
+        }
+        if (so != 0) {
+            if (g != 0) emit(e, so - 1, g - 1, -gm);
+            emit(e, so - 1, so - 1, gm);
+        }
+        if (d != 0) {
+            if (b != 0) emit(e, d - 1, b - 1, gmb);
+            if (so != 0) emit(e, d - 1, so - 1, -gmb);
+        }
+        if (so != 0) {
+            if (b != 0) emit(e, so - 1, b - 1, -gmb);
+            emit(e, so - 1, so - 1, gmb);
+        }
+        if (rhs) {
+            if (d != 0) rhs[d - 1] -= ieq;
+            if (so != 0) rhs[so - 1] += ieq;
+        }
+    }
+    /* gmin diagonal :2857-2860 */
+    for (int32_t i = 0; i < n; ++i) emit(e, i, i, gmin);
+}
+
+/* Sparse adjacency with O(1) membership (bitset) for graph elimination. */
+typedef struct {
+    int32_t nv;            /* vertices 0..nv-1 */
+    int64_t words;         /* per-row words */
+    uint64_t* bits;
+    ivec* nb;
+} egraph;
+static void eg_init(egraph* g, int32_t nv) {
+    g->nv = nv;
+    g->words = (nv + 63) / 64;
+    g->bits = (uint64_t*)xcalloc((size_t)(g->words * nv), 8);
+    g->nb = (ivec*)xcalloc((size_t)nv, sizeof(ivec));
+}
+static void eg_free(egraph* g) {
+    for (int32_t i = 0; i < g->nv; ++i) free(g->nb[i].a);
+    free(g->nb); free(g->bits);
+}
+static inline int eg_has(const egraph* g, int32_t u, int32_t w) {
+    return (int)((g->bits[(int64_t)u * g->words + (w >> 6)] >> (w & 63)) & 1u);
+}
+static inline void eg_add(egraph* g, int32_t u, int32_t w) {
+    if (u == w || eg_has(g, u, w)) return;
+    g->bits[(int64_t)u * g->words + (w >> 6)] |= (uint64_t)1 << (w & 63);
+    g->bits[(int64_t)w * g->words + (u >> 6)] |= (uint64_t)1 << (u & 63);
+    iv_push(&g->nb[u], w); iv_push(&g->nb[w], u);
+}
+static void eg_remove_vertex(egraph* g, int32_t v) {
+    ivec* nv = &g->nb[v];
+    for (int32_t k = 0; k < nv->n; ++k) {
+        int32_t u = nv->a[k];
+        ivec* nu = &g->nb[u];
+        for (int32_t j = 0; j < nu->n; ++j)
+            if (nu->a[j] == v) { nu->a[j] = nu->a[--nu->n]; break; }
+        g->bits[(int64_t)u * g->words + (v >> 6)] &= ~((uint64_t)1 << (v & 63));
+        g->bits[(int64_t)v * g->words + (u >> 6)] &= ~((uint64_t)1 << (u & 63));
+    }
+    nv->n = 0;
+}
+static void eg_cliquify(egraph* g, int32_t v) {
+    ivec* nv = &g->nb[v];
+    int32_t cnt = nv->n;
+    int32_t* tmp = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(cnt + 1));
+    memcpy(tmp, nv->a, sizeof(int32_t) * (size_t)cnt);
+    for (int32_t i = 0; i < cnt; ++i)
+        for (int32_t j = i + 1; j < cnt; ++j) eg_add(g, tmp[i], tmp[j]);
+    free(tmp);
+}
+static uint32_t eg_fill(const egraph* g, int32_t v) {
+    const ivec* nv = &g->nb[v];
+    uint32_t f = 0;
+    for (int32_t i = 0; i < nv->n; ++i)
+        for (int32_t j = i + 1; j < nv->n; ++j)
+            if (!eg_has(g, nv->a[i], nv->a[j])) ++f;
+    return f;
+}
+static int cmp_i32(const void* a, const void* b) {
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* a9 + a10: FactorGraph::fromTensorizedBlock (infrastructure/factor_graph.cpp
+ * :9-162) and TreewidthAnalyzer::analyze (infrastructure/treewidth_analyzer.cpp
+ * :14-141), restated literally: every step scans all remaining variables in
+ * ascending id order, fill = number of non-adjacent neighbour pairs, argmin
+ * with ties to the smallest id; branch-current ids start at numCircuitNodes
+ * and may collide with a node id (D4, kept as coded). */
+static void min_fill(const nr_netlist* nl, const orc_soa* s, orc_topo* t) {
+    const int32_t ncn = nl->num_nodes;
+    const int32_t nbranch = nl->n_vsrc;        /* + inductors (none in this ABI) */
+    const int32_t maxid = ncn + nbranch;       /* ids 0..maxid */
+    egraph g;
+    eg_init(&g, maxid + 1);
+    char* touched = (char*)xcalloc((size_t)maxid + 1, 1);
+    int32_t branch = ncn;
+#define FACTOR_BEGIN() int32_t fv[8]; int32_t nfv = 0
+#define FACTOR_ADD(x) do { if ((x) > 0) { fv[nfv++] = (x); touched[(x)] = 1; } } while (0)
+#define FACTOR_END() do { for (int32_t i_ = 0; i_ < nfv; ++i_) for (int32_t j_ = i_ + 1; j_ < nfv; ++j_) \
+        if (fv[i_] != fv[j_]) eg_add(&g, fv[i_], fv[j_]); } while (0)
+    for (int32_t i = 0; i < s->nD; ++i) { FACTOR_BEGIN(); FACTOR_ADD(s->da[i]); FACTOR_ADD(s->dc[i]); FACTOR_END(); }
+    for (int32_t i = 0; i < s->nM; ++i) {
+        FACTOR_BEGIN(); FACTOR_ADD(s->md[i]); FACTOR_ADD(s->mg[i]); FACTOR_ADD(s->ms[i]); FACTOR_ADD(s->mb[i]); FACTOR_END();
+    }
+    for (int32_t i = 0; i < nl->n_res; ++i) { FACTOR_BEGIN(); FACTOR_ADD(nl->res_n1[i]); FACTOR_ADD(nl->res_n2[i]); FACTOR_END(); }
+    for (int32_t i = 0; i < nl->n_cap; ++i) { FACTOR_BEGIN(); FACTOR_ADD(nl->cap_n1[i]); FACTOR_ADD(nl->cap_n2[i]); FACTOR_END(); }
+    for (int32_t i = 0; i < nl->n_vsrc; ++i) {
+        FACTOR_BEGIN(); FACTOR_ADD(nl->vsrc_pos[i]); FACTOR_ADD(nl->vsrc_neg[i]);
+        fv[nfv++] = branch++;            /* branch-current variable */
+        FACTOR_END();
+    }
+    for (int32_t i = 0; i < nl->n_isrc; ++i) { FACTOR_BEGIN(); FACTOR_ADD(nl->isrc_pos[i]); FACTOR_ADD(nl->isrc_neg[i]); FACTOR_END(); }
+#undef FACTOR_BEGIN
+#undef FACTOR_ADD
+#undef FACTOR_END
+    /* variables: sorted touched node ids, then branch ids (possibly colliding) */
+    int32_t nvars = 0;
+    int32_t* rem = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(maxid + nbranch + 2));
+    for (int32_t id = 1; id <= maxid; ++id)
+        if (id <= ncn && touched[id]) rem[nvars++] = id;
+    for (int32_t k = 0; k < nbranch; ++k) rem[nvars++] = ncn + k;
+    free(touched);
+    qsort(rem, (size_t)nvars, sizeof(int32_t), cmp_i32);
+    t->n_vars = nvars;
+    t->ordering_len = nvars;
+    t->ordering_raw = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(nvars + 1));
+    int32_t nrem = nvars;
+    uint32_t maxClique = 0;
+    for (int32_t step = 0; step < nvars; ++step) {
+        int32_t best = rem[0], bestpos = 0;
+        uint32_t bestFill = UINT32_MAX;
+        for (int32_t k = 0; k < nrem; ++k) {
+            int32_t v = rem[k];
+            uint32_t f = eg_fill(&g, v);
+            if (f < bestFill || (f == bestFill && v < best)) { bestFill = f; best = v; bestpos = k; }
+        }
+        /* std::find returns the first copy of `best` (rem is sorted) */
+        for (int32_t k = 0; k < nrem; ++k) if (rem[k] == best) { bestpos = k; break; }
+        t->ordering_raw[step] = best;
+        uint32_t clique = (uint32_t)g.nb[best].n + 1;
+        if (clique > maxClique) maxClique = clique;
+        eg_cliquify(&g, best);
+        eg_remove_vertex(&g, best);
+        memmove(rem + bestpos, rem + bestpos + 1, sizeof(int32_t) * (size_t)(nrem - bestpos - 1));
+        --nrem;
+    }
+    t->treewidth = maxClique > 0 ? (int32_t)maxClique - 1 : 0;
+    free(rem);
+    eg_free(&g);
+}
+
+/* D3/D4 projection of the Min-Fill ordering onto matrix rows: keep the first
+ * occurrence of each node id 1..n, drop ids above n, row = id - 1; rows that
+ * no device touches (none in practice) are appended in ascending order. */
+static void project_ordering(orc_topo* t) {
+    const int32_t n = t->n;
+    char* seen = (char*)xcalloc((size_t)n + 1, 1);
+    int32_t k = 0;
+    t->perm = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    t->iperm = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    for (int32_t i = 0; i < t->ordering_len; ++i) {
+        int32_t id = t->ordering_raw[i];
+        if (id >= 1 && id <= n && !seen[id]) { seen[id] = 1; t->perm[k++] = id - 1; }
+    }
+    for (int32_t id = 1; id <= n; ++id) if (!seen[id]) t->perm[k++] = id - 1;
+    for (int32_t i = 0; i < n; ++i) t->iperm[t->perm[i]] = i;
+    free(seen);
+}
+
+/* a11: symbolicFactorize (math/sparse_lu.cpp:14-128) restated. */
+static void symbolic(orc_topo* t) {
+    const int32_t n = t->n;
+    egraph g;
+    eg_init(&g, n);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t k = t->row_ptr[r]; k < t->row_ptr[r + 1]; ++k)
+            if (t->col_idx[k] != r) eg_add(&g, r, t->col_idx[k]);
+    ivec* rows = (ivec*)xcalloc((size_t)n, sizeof(ivec));
+    for (int32_t i = 0; i < n; ++i) iv_push(&rows[i], i);
+    int32_t fill = 0;
+    for (int32_t step = 0; step < n; ++step) {
+        int32_t v = t->perm[step];
+        ivec* nv = &g.nb[v];
+        for (int32_t k = 0; k < nv->n; ++k) {
+            int32_t pc = t->iperm[nv->a[k]];
+            iv_push(&rows[step], pc);
+            iv_push(&rows[pc], step);
+        }
+        int32_t cnt = nv->n;
+        int32_t* tmp = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(cnt + 1));
+        memcpy(tmp, nv->a, sizeof(int32_t) * (size_t)cnt);
+        for (int32_t i = 0; i < cnt; ++i)
+            for (int32_t j = i + 1; j < cnt; ++j)
+                if (!eg_has(&g, tmp[i], tmp[j])) { eg_add(&g, tmp[i], tmp[j]); ++fill; }
+        free(tmp);
+        eg_remove_vertex(&g, v);
+    }
+    eg_free(&g);
+    t->fill_in = fill;
+    t->frow_ptr = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    int64_t tot = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        qsort(rows[i].a, (size_t)rows[i].n, sizeof(int32_t), cmp_i32);
+        int32_t u = 0;   /* dedupe (the reference uses a set per row) */
+        for (int32_t k = 0; k < rows[i].n; ++k)
+            if (k == 0 || rows[i].a[k] != rows[i].a[k - 1]) rows[i].a[u++] = rows[i].a[k];
+        rows[i].n = u;
+        tot += u;
+    }
+    t->nnz_f = (int32_t)tot;
+    t->fcol_idx = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(tot + 1));
+    t->fdiag = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    int32_t p = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        t->frow_ptr[i] = p;
+        for (int32_t k = 0; k < rows[i].n; ++k) {
+            if (rows[i].a[k] == i) t->fdiag[i] = p;
+            t->fcol_idx[p++] = rows[i].a[k];
+        }
+        free(rows[i].a);
+    }
+    t->frow_ptr[n] = p;
+    free(rows);
+}
+
+/* a8: createCsr / buildPattern (math/linalg.h:281-431): sort the triplet
+ * coordinates by (row, col), merge duplicates; the stamp map is each
+ * triplet's merged slot (assembleFast's position_map lookup, :441-465). */
+static int64_t* g_sort_keys;
+static int cmp_key_idx(const void* a, const void* b) {
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    int64_t kx = g_sort_keys[x], ky = g_sort_keys[y];
+    if (kx != ky) return (kx > ky) - (kx < ky);
+    return (x > y) - (x < y);
+}
+
+int orc_topology_build(const nr_netlist* nl, orc_topo** out) {
+    if (!nl || nl->num_nodes <= 0) return NR_ERR_INVALID;
+    orc_topo* t = (orc_topo*)xcalloc(1, sizeof(orc_topo));
+    t->n = nl->num_nodes;
+    t->soa = orc_tensorize(nl);
+    emitter e;
+    memset(&e, 0, sizeof(e));
+    stamp_all(nl, t->soa, NULL, 0.0, &e);
+    t->n_stamps = e.rows.n;
+    t->st_row = e.rows.a; t->st_col = e.cols.a;
+    const int32_t S = t->n_stamps, n = t->n;
+    int64_t* keys = (int64_t*)xmalloc(sizeof(int64_t) * (size_t)S);
+    int32_t* idx = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)S);
+    for (int32_t k = 0; k < S; ++k) { keys[k] = ((int64_t)t->st_row[k] << 32) | (uint32_t)t->st_col[k]; idx[k] = k; }
+    g_sort_keys = keys;
+    qsort(idx, (size_t)S, sizeof(int32_t), cmp_key_idx);
+    g_sort_keys = NULL;
+    t->stamp_slot = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)S);
+    t->row_ptr = (int32_t*)xcalloc((size_t)n + 1, 4);
+    t->col_idx = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)S);
+    int32_t nnz = 0;
+    int64_t last = -1;
+    for (int32_t k = 0; k < S; ++k) {
+        int32_t j = idx[k];
+        if (keys[j] != last) {
+            t->col_idx[nnz] = t->st_col[j];
+            t->row_ptr[t->st_row[j] + 1]++;
+            ++nnz;
+            last = keys[j];
+        }
+        t->stamp_slot[j] = nnz - 1;
+    }
+    for (int32_t i = 0; i < n; ++i) t->row_ptr[i + 1] += t->row_ptr[i];
+    t->nnz_a = nnz;
+    free(keys); free(idx);
+    min_fill(nl, t->soa, t);
+    project_ordering(t);
+    symbolic(t);
+    *out = t;
+    return NR_OK;
+}
+
+void orc_topology_free(orc_topo* t) {
+    if (!t) return;
+    orc_soa_free(t->soa);
+    free(t->st_row); free(t->st_col); free(t->row_ptr); free(t->col_idx); free(t->stamp_slot);
+    free(t->ordering_raw); free(t->perm); free(t->iperm);
+    free(t->frow_ptr); free(t->fcol_idx); free(t->fdiag);
+    free(t);
+}
+
+void orc_topology_info(const orc_topo* t, int32_t* info) {
+    info[0] = t->n; info[1] = t->nnz_a; info[2] = t->nnz_f; info[3] = t->n_stamps;
+    info[4] = t->treewidth; info[5] = t->ordering_len; info[6] = t->fill_in; info[7] = t->n_vars;
+}
+
+int64_t orc_topology_export(const orc_topo* t, int32_t what, int32_t* out, int64_t cap) {
+    const int32_t* src = NULL;
+    int64_t cnt = 0;
+    switch (what) {
+    case NR_EXPORT_ROW_PTR: src = t->row_ptr; cnt = t->n + 1; break;
+    case NR_EXPORT_COL_IDX: src = t->col_idx; cnt = t->nnz_a; break;
+    case NR_EXPORT_STAMP_SLOT: src = t->stamp_slot; cnt = t->n_stamps; break;
+    case NR_EXPORT_ORDERING_RAW: src = t->ordering_raw; cnt = t->ordering_len; break;
+    case NR_EXPORT_PERM: src = t->perm; cnt = t->n; break;
+    case NR_EXPORT_FILL_ROW_PTR: src = t->frow_ptr; cnt = t->n + 1; break;
+    case NR_EXPORT_FILL_COL_IDX: src = t->fcol_idx; cnt = t->nnz_f; break;
+    case NR_EXPORT_DIODE_ORDER: src = t->soa->d_src; cnt = t->soa->nD; break;
+    case NR_EXPORT_MOS_ORDER: src = t->soa->m_src; cnt = t->soa->nM; break;
+    case ORC_EXPORT_STAMP_ROW: src = t->st_row; cnt = t->n_stamps; break;
+    case ORC_EXPORT_STAMP_COL: src = t->st_col; cnt = t->n_stamps; break;
+    default: return -1;
+    }
+    if (cnt > cap) return -1;
+    if (cnt) memcpy(out, src, sizeof(int32_t) * (size_t)cnt);
+    return cnt;
+}
+
+/* ------------------------------------------------------------------------ */
+/* stamping + assembly of one iteration                                     */
+/* ------------------------------------------------------------------------ */
+static int stamp_with_soa(const nr_netlist* nl, const orc_soa* s, const orc_topo* t,
+                          const double* v, double gmin, double* vals, double* rhs) {
+    emitter e;
+    memset(&e, 0, sizeof(e));
+    e.vals = vals;
+    e.rhs = rhs;
+    memset(rhs, 0, sizeof(double) * (size_t)nl->num_nodes);
+    stamp_all(nl, s, v, gmin, &e);
+    return e.nv == t->n_stamps ? NR_OK : NR_ERR_TOPOLOGY;
+}
+
+int orc_stamp(const nr_netlist* nl, const orc_topo* t, const double* v,
+              double gmin, double* vals, double* rhs) {
+    orc_soa* s = orc_tensorize(nl);
+    int rc = stamp_with_soa(nl, s, t, v, gmin, vals, rhs);
+    orc_soa_free(s);
+    return rc;
+}
+
+int orc_assemble(const orc_topo* t, const double* vals, double* a_vals) {
+    memset(a_vals, 0, sizeof(double) * (size_t)t->nnz_a);
+    for (int32_t k = 0; k < t->n_stamps; ++k) a_vals[t->stamp_slot[k]] += vals[k];
+    return NR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a12: linear solves                                                       */
+/* ------------------------------------------------------------------------ */
+int orc_lu_dense(int32_t n, const int32_t* rp, const int32_t* ci,
+                 const double* vals, const double* rhs, double* x,
+                 double* smallest_pivot) {
+    double* A = (double*)xcalloc((size_t)n * (size_t)n, 8);
+    double* b = (double*)xmalloc(sizeof(double) * (size_t)n);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t k = rp[r]; k < rp[r + 1]; ++k) A[(size_t)r * n + ci[k]] += vals[k];
+    memcpy(b, rhs, sizeof(double) * (size_t)n);
+    double smallest = 1e300;
+    for (int32_t i = 0; i < n; ++i) {
+        double maxVal = 0.0;
+        int32_t maxRow = i;
+        for (int32_t k = i; k < n; ++k) {
+            double val = fabs(A[(size_t)k * n + i]);
+            if (val > maxVal) { maxVal = val; maxRow = k; }
+            else if (val == maxVal && k < maxRow) maxRow = k;
+        }
+        if (maxVal < smallest) smallest = maxVal;
+        if (maxVal < 1e-25) {
+            for (int32_t k = 0; k < n; ++k) x[k] = 0.0;
+            free(A); free(b);
+            if (smallest_pivot) *smallest_pivot = smallest;
+            return NR_STATUS_SINGULAR;
+        }
+        if (maxRow != i) {
+            for (int32_t k = 0; k < n; ++k) {
+                double tmp = A[(size_t)i * n + k];
+                A[(size_t)i * n + k] = A[(size_t)maxRow * n + k];
+                A[(size_t)maxRow * n + k] = tmp;
+            }
+            double tb = b[i]; b[i] = b[maxRow]; b[maxRow] = tb;
+        }
+        const double piv = A[(size_t)i * n + i];
+        for (int32_t j = i + 1; j < n; ++j) {
+            double* rj = &A[(size_t)j * n];
+            rj[i] /= piv;
+            const double l = rj[i];
+            const double* ri = &A[(size_t)i * n];
+            if (l != 0.0)
+                for (int32_t k = i + 1; k < n; ++k) rj[k] -= l * ri[k];
+        }
+    }
+    for (int32_t i = 0; i < n; ++i) {
+        double y = b[i];
+        for (int32_t j = 0; j < i; ++j) y -= A[(size_t)i * n + j] * b[j];
+        b[i] = y;
+    }
+    for (int32_t i = n - 1; i >= 0; --i) {
+        double xi = b[i];
+        for (int32_t j = i + 1; j < n; ++j) xi -= A[(size_t)i * n + j] * x[j];
+        x[i] = xi / A[(size_t)i * n + i];
+    }
+    if (smallest_pivot) *smallest_pivot = smallest;
+    free(A); free(b);
+    return NR_OK;
+}
+
+/* Row-oriented (IKJ) LU without pivoting on the filled pattern in permuted
+ * space; the pattern is closed under elimination, so every update lands on
+ * a pattern position. */
+int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
+                  double* x, double* smallest_pivot) {
+    const int32_t n = t->n;
+    const int32_t *fr = t->frow_ptr, *fc = t->fcol_idx, *fd = t->fdiag;
+    double* F = (double*)xcalloc((size_t)t->nnz_f, 8);
+    double* w = (double*)xcalloc((size_t)n, 8);
+    double* y = (double*)xmalloc(sizeof(double) * (size_t)n);
+    /* scatter A into the permuted pattern */
+    for (int32_t r = 0; r < n; ++r) {
+        int32_t pr = t->iperm[r];
+        for (int32_t k = t->row_ptr[r]; k < t->row_ptr[r + 1]; ++k) {
+            int32_t pc = t->iperm[t->col_idx[k]];
+            w[pc] = a_vals[k];
+        }
+        for (int32_t k = fr[pr]; k < fr[pr + 1]; ++k) { F[k] = w[fc[k]]; }
+        for (int32_t k = t->row_ptr[r]; k < t->row_ptr[r + 1]; ++k) w[t->iperm[t->col_idx[k]]] = 0.0;
+    }
+    double smallest = 1e300;
+    int rc = NR_OK;
+    for (int32_t i = 0; i < n && rc == NR_OK; ++i) {
+        for (int32_t k = fr[i]; k < fr[i + 1]; ++k) w[fc[k]] = F[k];
+        for (int32_t k = fr[i]; k < fd[i]; ++k) {
+            int32_t j = fc[k];
+            double l = w[j] / F[fd[j]];
+            w[j] = l;
+            for (int32_t q = fd[j] + 1; q < fr[j + 1]; ++q) w[fc[q]] -= l * F[q];
+        }
+        for (int32_t k = fr[i]; k < fr[i + 1]; ++k) { F[k] = w[fc[k]]; w[fc[k]] = 0.0; }
+        double piv = fabs(F[fd[i]]);
+        if (piv < smallest) smallest = piv;
+        if (!(piv >= 1e-25)) rc = NR_STATUS_SINGULAR;
+    }
+    if (rc == NR_OK) {
+        for (int32_t i = 0; i < n; ++i) {
+            double s = rhs[t->perm[i]];
+            for (int32_t k = fr[i]; k < fd[i]; ++k) s -= F[k] * y[fc[k]];
+            y[i] = s;
+        }
+        for (int32_t i = n - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int32_t k = fd[i] + 1; k < fr[i + 1]; ++k) s -= F[k] * y[fc[k]];
+            y[i] = s / F[fd[i]];
+        }
+        for (int32_t i = 0; i < n; ++i) x[t->perm[i]] = y[i];
+    } else {
+        for (int32_t i = 0; i < n; ++i) x[i] = 0.0;
+    }
+    if (smallest_pivot) *smallest_pivot = smallest;
+    free(F); free(w); free(y);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a14: seedInitialGuess (solvers/circuitsim_solver.cpp:32-116)             */
+/* ------------------------------------------------------------------------ */
+void orc_seed(const nr_netlist* nl, double* v) {
+    const int32_t N = nl->num_nodes;
+    for (int32_t i = 0; i < N; ++i) v[i] = 0.0;
+    for (int32_t i = 0; i < nl->n_vsrc; ++i) {
+        int32_t p = nl->vsrc_pos[i];
+        if (p > 0 && p <= N) v[p - 1] = nl->vsrc_volts[i];
+    }
+    for (int32_t i = 0; i < nl->n_diode; ++i) {
+        int32_t a = nl->dio_anode[i];
+        if (a > 0 && a <= N) v[a - 1] = dmax_ref(v[a - 1], 0.6);
+    }
+    double vSupply = 5.0;
+    for (int32_t i = 0; i < nl->n_vsrc; ++i)
+        if (nl->vsrc_volts[i] > vSupply) vSupply = nl->vsrc_volts[i];
+    for (int32_t i = 0; i < nl->n_isrc; ++i) {
+        int32_t p = nl->isrc_pos[i];
+        if (p <= 0 || p > N) continue;
+        if (v[p - 1] == 0.0) v[p - 1] = vSupply * 0.5;   /* no BJTs on this path */
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a18: pnjlim / compute_vcrit (physics/device_physics.h:132-179) and       */
+/* applyPnJunctionLimits (solvers/circuitsim_solver.cpp:118-169)            */
+/* ------------------------------------------------------------------------ */
+double orc_pnjlim(double vnew, double vold, double vt, double vcrit) {
+    double vlim = vnew;
+    if (vold > 30.0 * vt && vnew > 30.0 * vt) return vnew;
+    if (vold < -5.0 * vt && vnew < -5.0 * vt) return vnew;
+    if (vnew > vcrit && fabs(vnew - vold) > 2.0 * vt) {
+        if (vold > 0.0) {
+            double arg = 1.0 + (vnew - vold) / vt;
+            if (arg > 0.0) vlim = vold + vt * log(arg);
+            else vlim = vcrit;
+        } else {
+            vlim = vt * log(vnew / vt);
+        }
+    } else if (vnew < -5.0 * vt) {
+        vlim = dmax_ref(vnew, vold - 5.0 * vt);
+    }
+    return vlim;
+}
+
+double orc_vcrit(double vt, double is) { return vt * log(vt / (sqrt(2.0) * is)); }
+
+static void apply_pn_limits_soa(const nr_netlist* nl, const orc_soa* s,
+                                double* v_new, const double* v_old) {
+    const int32_t N = nl->num_nodes;
+    const double ts = nl->ambient_temp_K / 300.15;
+    for (int32_t i = 0; i < s->nD; ++i) {
+        double Vt = s->dVt[i], Is = s->dIs[i];
+        Vt *= ts;
+        Is *= pow(ts, 3.0) * exp(1.11 / (Vt / ts) - 1.11 / Vt);
+        int32_t p = s->da[i] - 1, q = s->dc[i] - 1;
+        if (p < 0 && q < 0) continue;
+        if (p >= N || q >= N) continue;
+        double vcrit = orc_vcrit(Vt, Is);
+        double vjo = (p >= 0 ? v_old[p] : 0.0) - (q >= 0 ? v_old[q] : 0.0);
+        double vjn = (p >= 0 ? v_new[p] : 0.0) - (q >= 0 ? v_new[q] : 0.0);
+        double vl = orc_pnjlim(vjn, vjo, Vt, vcrit);
+        double corr = vjn - vl;
+        if (fabs(corr) > 1e-12) {
+            if (p >= 0 && q >= 0) { v_new[p] -= corr * 0.5; v_new[q] += corr * 0.5; }
+            else if (p >= 0) v_new[p] -= corr;
+            else if (q >= 0) v_new[q] += corr;
+        }
+    }
+}
+
+void orc_apply_pn_limits(const nr_netlist* nl, const orc_topo* t,
+                         double* v_new, const double* v_old) {
+    (void)t;
+    orc_soa* s = orc_tensorize(nl);
+    apply_pn_limits_soa(nl, s, v_new, v_old);
+    orc_soa_free(s);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a19: calculatePhysicalResiduals (physics/circuitsim.h:1250-1671), DC     */
+/* terms, netlist order.  D7 fix: the MOSFET term uses the SoA parameters   */
+/* and the generic-kernel current (same equations as the stamp).            */
+/* ------------------------------------------------------------------------ */
+static double diode_i_clamped(double vd, double Is, double N, double Vt) {
+    double n_vt = N * Vt;
+    double arg = vd / n_vt;
+    if (arg > 30.0) {
+        double em = exp(30.0);
+        double imax = Is * (em - 1.0);
+        double gmax = (Is / n_vt) * em;
+        double delta = vd - (n_vt * 30.0);
+        return imax + delta * gmax;
+    }
+    double ev = exp(arg);
+    return Is * (ev - 1.0);
+}
+
+static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
+                      double* res) {
+    const int32_t n = nl->num_nodes;
+    for (int32_t i = 0; i < n; ++i) res[i] = 0.0;
+#define ADDRES(node, val) do { int32_t nd_ = (node); if (nd_ > 0 && nd_ <= n) res[nd_ - 1] += (val); } while (0)
+    for (int32_t i = 0; i < nl->n_res; ++i) {
+        double ir = (volt(v, n, nl->res_n1[i]) - volt(v, n, nl->res_n2[i])) / nl->res_ohms[i];
+        ADDRES(nl->res_n1[i], -ir); ADDRES(nl->res_n2[i], ir);
+    }
+    for (int32_t i = 0; i < nl->n_vsrc; ++i) {
+        double isrc = (nl->vsrc_volts[i] - (volt(v, n, nl->vsrc_pos[i]) - volt(v, n, nl->vsrc_neg[i]))) / 1e-3;
+        ADDRES(nl->vsrc_pos[i], isrc); ADDRES(nl->vsrc_neg[i], -isrc);
+    }
+    for (int32_t i = 0; i < nl->n_isrc; ++i) {
+        ADDRES(nl->isrc_pos[i], -nl->isrc_amps[i]); ADDRES(nl->isrc_neg[i], nl->isrc_amps[i]);
+    }
+    for (int32_t i = 0; i < nl->n_diode; ++i) {
+        double vd = volt(v, n, nl->dio_anode[i]) - volt(v, n, nl->dio_cathode[i]);
+        double id = diode_i_clamped(vd, nl->dio_is[i], nl->dio_n[i], nl->dio_vt[i]);
+        ADDRES(nl->dio_anode[i], -id); ADDRES(nl->dio_cathode[i], id);
+    }
+    if (s->nM > 0) {
+        int32_t* pos = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)s->nM);
+        for (int32_t p = 0; p < s->nM; ++p) pos[s->m_src[p]] = p;
+        for (int32_t i = 0; i < nl->n_mos; ++i) {
+            int32_t p = pos[i];
+            double vgs, vds, ids, gm, gds;
+            mos_one(volt(v, n, s->mg[p]), volt(v, n, s->md[p]), volt(v, n, s->ms[p]), s->mW[p], s->mL[p],
+                    s->mKp[p], s->mVth[p], s->mLam[p], s->mP[p], &vgs, &vds, &ids, &gm, &gds);
+            ADDRES(nl->mos_d[i], -ids); ADDRES(nl->mos_s[i], ids);
+        }
+        free(pos);
+    }
+#undef ADDRES
+    double mx = 0.0;
+    for (int32_t i = 0; i < n; ++i) mx = dmax_ref(mx, fabs(res[i]));
+    return mx;
+}
+
+double orc_kcl(const nr_netlist* nl, const orc_topo* t, const double* v, double* per_node) {
+    (void)t;
+    orc_soa* s = orc_tensorize(nl);
+    double* res = per_node ? per_node : (double*)xmalloc(sizeof(double) * (size_t)nl->num_nodes);
+    double m = kcl_soa(nl, s, v, res);
+    if (!per_node) free(res);
+    orc_soa_free(s);
+    return m;
+}
+
+/* a20: checkConvergence (physics/circuitsim.h:1218-1241) */
+int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
+                          double kcl, double kcl_tol) {
+    if (kcl < 1e-9) return 1;
+    for (int32_t i = 0; i < n; ++i) {
+        double diff = fabs(v_new[i] - v_old[i]);
+        double limit = 1e-3 * dmax_ref(fabs(v_new[i]), fabs(v_old[i])) + 1e-6;
+        if (diff > limit) return 0;
+    }
+    if (kcl > kcl_tol) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a15: solveDC (solvers/circuitsim_solver.cpp:176-555)                     */
+/* ------------------------------------------------------------------------ */
+void orc_dc_opts_default(orc_dc_opts* o) {
+    o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_SPARSE;
+    o->start_step = 0; o->use_seed = 0;
+}
+
+int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opts,
+                 double* v_io, orc_dc_out* out) {
+    orc_dc_opts o;
+    if (opts) o = *opts; else orc_dc_opts_default(&o);
+    memset(out, 0, sizeof(*out));
+    const int32_t n = nl->num_nodes;
+    if (n <= 0) return NR_ERR_INVALID;
+    orc_topo* own = NULL;
+    if (!t) { int rc = orc_topology_build(nl, &own); if (rc) return rc; t = own; }
+    orc_soa* s = orc_tensorize(nl);
+    double* v = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* vn = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* rhs = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* res = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    if (o.use_seed) memcpy(v, v_io, sizeof(double) * (size_t)n);
+    else orc_seed(nl, v);
+    int converged = 0, singular = 0;
+    int32_t total = 0, ref_total = 0, steps_done = 0;
+    double final_kcl = 0.0;
+    for (int32_t k = o.start_step; k < o.gmin_steps; ++k) {
+        double gmin = pow(10.0, -(3.0 + k * 0.5));
+        if (gmin < 1e-12) gmin = 1e-12;
+        int gconv = 0, sing_step = 0;
+        int32_t it = 0;
+        for (; it < o.max_iter; ++it) {
+            stamp_with_soa(nl, s, t, v, gmin, vals, rhs);
+            orc_assemble(t, vals, av);
+            int rc = (o.lu_kind == ORC_LU_DENSE)
+                         ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
+                         : orc_lu_sparse(t, av, rhs, vn, NULL);
+            if (rc != NR_OK) {
+                final_kcl = 1e9;
+                singular = 1; sing_step = 1;
+                ref_total += it + 1;           /* D15: counted here and below */
+                break;
+            }
+            for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
+            apply_pn_limits_soa(nl, s, vn, v);
+            double kcl = kcl_soa(nl, s, vn, res);
+            if (it > 0 && kcl >= final_kcl) {
+                for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
+                kcl = kcl_soa(nl, s, vn, res);
+                if (kcl >= final_kcl) {
+                    for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
+                    kcl = kcl_soa(nl, s, vn, res);
+                }
+            }
+            final_kcl = kcl;
+            double tol = (k == o.gmin_steps - 1) ? 1e-12 : 1.0;
+            if (orc_check_convergence(n, vn, v, kcl, tol)) {
+                memcpy(v, vn, sizeof(double) * (size_t)n);
+                gconv = 1;
+                break;
+            }
+            memcpy(v, vn, sizeof(double) * (size_t)n);
+        }
+        steps_done++;
+        int32_t true_it = gconv ? it + 1 : (sing_step ? it + 1 : it);
+        if (k < 32) out->iters_per_step[k] = true_it;
+        total += true_it;
+        ref_total += it + 1;
+        if (k == o.gmin_steps - 1 && gconv) { converged = 1; break; }
+        if (!gconv && !(final_kcl < 1.0)) break;
+    }
+    double fk = kcl_soa(nl, s, v, res);
+    out->iterations = total;
+    out->ref_iterations = ref_total;
+    out->status = converged ? NR_STATUS_CONVERGED : (singular ? NR_STATUS_SINGULAR : NR_STATUS_NOT_CONVERGED);
+    out->gmin_steps_done = steps_done;
+    out->kcl = fk;
+    memcpy(v_io, v, sizeof(double) * (size_t)n);
+    free(v); free(vn); free(rhs); free(res); free(vals); free(av);
+    orc_soa_free(s);
+    if (own) orc_topology_free(own);
+    return NR_OK;
+}
+
+int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
+                       const double* const* fields, const orc_dc_opts* opts,
+                       int32_t threads, double* v_out, orc_dc_out* outs) {
+    if (!t || B <= 0) return NR_ERR_INVALID;
+    const int32_t n = nl->num_nodes;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int32_t b = 0; b < B; ++b) {
+        nr_netlist inst = *nl;
+        if (fields) {
+            if (fields[NR_FIELD_RES_OHMS]) inst.res_ohms = fields[NR_FIELD_RES_OHMS] + (size_t)b * nl->n_res;
+            if (fields[NR_FIELD_DIO_IS]) inst.dio_is = fields[NR_FIELD_DIO_IS] + (size_t)b * nl->n_diode;
+            if (fields[NR_FIELD_MOS_KP]) inst.mos_kp = fields[NR_FIELD_MOS_KP] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_MOS_VTO]) inst.mos_vto = fields[NR_FIELD_MOS_VTO] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
+            if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
+            if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+        }
+        orc_solve_dc(&inst, t, opts, v_out + (size_t)b * n, &outs[b]);
+    }
+    return NR_OK;
+}
+
+/* SoA arrays of tensorizeNetlist (same layout as the reference harness's
+ * ref_tensorize, for the fixture comparison). */
+int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
+                         double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,
+                         double* m_w, double* m_l, double* m_kp, double* m_vth, double* m_lam,
+                         int32_t* m_pmos) {
+    orc_soa* s = orc_tensorize(nl);
+    for (int32_t i = 0; i < s->nD; ++i) {
+        d_a[i] = s->da[i]; d_c[i] = s->dc[i]; d_is[i] = s->dIs[i]; d_n[i] = s->dN[i]; d_vt[i] = s->dVt[i];
+    }
+    for (int32_t i = 0; i < s->nM; ++i) {
+        m_d[i] = s->md[i]; m_g[i] = s->mg[i]; m_s[i] = s->ms[i]; m_b[i] = s->mb[i];
+        m_w[i] = s->mW[i]; m_l[i] = s->mL[i]; m_kp[i] = s->mKp[i]; m_vth[i] = s->mVth[i];
+        m_lam[i] = s->mLam[i]; m_pmos[i] = s->mP[i];
+    }
+    orc_soa_free(s);
+    return 0;
+}
+
+/* diode_model_pair current used by the KCL audit */
+double orc_diode_i_clamped(double vd, double Is, double N, double Vt) { return diode_i_clamped(vd, Is, N, Vt); }

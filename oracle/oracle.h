@@ -1,0 +1,115 @@
+/*
+ * oracle.h — CPU FP64 restatement of the reference engine's Newton-Raphson
+ * DC path.  TEST INFRASTRUCTURE ONLY: it is the checker that tests, the smoke
+ * entry and the CPU-baseline leg of the benchmark compare the HIP library
+ * against.  The product library never links or calls it.
+ *
+ * Every function cites the reference file:line it restates.  The netlist
+ * format is the library's public `nr_netlist` (include/karam_nr.h).
+ */
+#ifndef KARAM_ORACLE_H
+#define KARAM_ORACLE_H
+
+#include <stdint.h>
+#include "../include/karam_nr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_soa orc_soa;
+typedef struct orc_topo orc_topo;
+
+/* --- a2: tensorizeNetlist ------------------------------------------------ */
+orc_soa* orc_tensorize(const nr_netlist* nl);
+void orc_soa_free(orc_soa* s);
+
+/* --- a4/a5: generic device kernels --------------------------------------- */
+void orc_eval_diodes(int32_t count, const int32_t* a, const int32_t* c,
+                     const double* Is, const double* N, const double* Vt,
+                     const double* v, int32_t nv,
+                     double* vd, double* id, double* gd);
+void orc_eval_mosfets(int32_t count, const int32_t* d, const int32_t* g,
+                      const int32_t* s, const double* W, const double* L,
+                      const double* Kp, const double* Vth, const double* lam,
+                      const int32_t* pmos, const double* v, int32_t nv,
+                      double* vgs, double* vds, double* ids, double* gm,
+                      double* gds);
+
+/* --- a6..a11: topology (pattern, stamp map, Min-Fill, symbolic) ---------- */
+int orc_topology_build(const nr_netlist* nl, orc_topo** out);
+void orc_topology_free(orc_topo* t);
+/* info[0..7] = n, nnz_a, nnz_f, n_stamps, treewidth, ordering_len, fill_in,
+ * n_vars */
+void orc_topology_info(const orc_topo* t, int32_t* info);
+/* `what` uses the NR_EXPORT_* codes of karam_nr.h (plus the two oracle-only
+ * codes below). Returns count or -1. */
+enum { ORC_EXPORT_STAMP_ROW = 100, ORC_EXPORT_STAMP_COL = 101 };
+int64_t orc_topology_export(const orc_topo* t, int32_t what, int32_t* out,
+                            int64_t capacity);
+
+/* Triplet values and rhs of one NR iteration (stampSoABlock + gmin), in the
+ * reference emission order: vals[n_stamps], rhs[n]. */
+int orc_stamp(const nr_netlist* nl, const orc_topo* t, const double* v,
+              double gmin, double* vals, double* rhs);
+/* CSR values of A from triplet values via the stamp map (assembleFast). */
+int orc_assemble(const orc_topo* t, const double* vals, double* a_vals);
+
+/* --- a12: linear solves --------------------------------------------------- */
+/* solveLU_Pivoted restated (math/linalg.h:742-854): dense, partial pivot.
+ * Returns 0 ok, 2 singular. */
+int orc_lu_dense(int32_t n, const int32_t* rp, const int32_t* ci,
+                 const double* vals, const double* rhs, double* x,
+                 double* smallest_pivot);
+/* Same-pattern sparse LU on the symbolic fill pattern, no pivoting. */
+int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
+                  double* x, double* smallest_pivot);
+
+/* --- a14, a18, a19, a20 ---------------------------------------------------- */
+void orc_seed(const nr_netlist* nl, double* v);
+double orc_pnjlim(double vnew, double vold, double vt, double vcrit);
+double orc_vcrit(double vt, double is);
+void orc_apply_pn_limits(const nr_netlist* nl, const orc_topo* t,
+                         double* v_new, const double* v_old);
+double orc_kcl(const nr_netlist* nl, const orc_topo* t, const double* v,
+               double* per_node);
+int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
+                          double kcl, double kcl_tol);
+
+/* --- a15: solveDC ---------------------------------------------------------- */
+enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1 };
+typedef struct orc_dc_opts {
+    int32_t max_iter;     /* 50 */
+    int32_t gmin_steps;   /* 19 */
+    int32_t lu_kind;      /* ORC_LU_* */
+    int32_t start_step;   /* GMIN step to start at (seeded start) */
+    int32_t use_seed;     /* 1: v_inout holds the start vector */
+} orc_dc_opts;
+typedef struct orc_dc_out {
+    int32_t iterations;      /* true NR iteration count */
+    int32_t ref_iterations;  /* reference SolverStats accounting */
+    int32_t status;          /* NR_STATUS_* */
+    int32_t gmin_steps_done;
+    double kcl;
+    int32_t iters_per_step[32];
+} orc_dc_out;
+void orc_dc_opts_default(orc_dc_opts* o);
+int orc_solve_dc(const nr_netlist* nl, const orc_topo* t,
+                 const orc_dc_opts* opts, double* v_inout, orc_dc_out* out);
+
+/* Batch over instances: fields[NR_FIELD_*] point at [B][count] arrays (or
+ * NULL for the template value).  OpenMP over instances with `threads`. */
+int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
+                       const double* const* fields, const orc_dc_opts* opts,
+                       int32_t threads, double* v_out, orc_dc_out* outs);
+
+int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
+                         double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,
+                         double* m_w, double* m_l, double* m_kp, double* m_vth, double* m_lam,
+                         int32_t* m_pmos);
+double orc_diode_i_clamped(double vd, double Is, double N, double Vt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

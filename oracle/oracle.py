@@ -1,0 +1,226 @@
+"""ctypes binding of the C oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY — the checker.  Imported by tests/, by
+__graft_entry__.smoke() and by bench.py's CPU-baseline leg, never by the
+product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from karam_amd import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+class OrcDcOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("lu_kind", C.c_int32),
+                ("start_step", C.c_int32), ("use_seed", C.c_int32)]
+
+
+class OrcDcOut(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
+                ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
+                ("iters_per_step", C.c_int32 * 32)]
+
+
+LU_DENSE = 0
+LU_SPARSE = 1
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.orc_topology_build.argtypes = [C.POINTER(_abi.NrNetlist), C.POINTER(C.c_void_p)]
+        L.orc_topology_free.argtypes = [C.c_void_p]
+        L.orc_topology_info.argtypes = [C.c_void_p, _i32p]
+        L.orc_topology_export.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64]
+        L.orc_topology_export.restype = C.c_int64
+        L.orc_stamp.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, C.c_double, _f64p, _f64p]
+        L.orc_assemble.argtypes = [C.c_void_p, _f64p, _f64p]
+        L.orc_lu_dense.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_lu_sparse.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_seed.argtypes = [C.POINTER(_abi.NrNetlist), _f64p]
+        L.orc_pnjlim.argtypes = [C.c_double] * 4
+        L.orc_pnjlim.restype = C.c_double
+        L.orc_vcrit.argtypes = [C.c_double, C.c_double]
+        L.orc_vcrit.restype = C.c_double
+        L.orc_apply_pn_limits.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, _f64p]
+        L.orc_kcl.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, _f64p]
+        L.orc_kcl.restype = C.c_double
+        L.orc_check_convergence.argtypes = [C.c_int32, _f64p, _f64p, C.c_double, C.c_double]
+        L.orc_eval_diodes.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int32,
+                                      _f64p, _f64p, _f64p]
+        L.orc_eval_mosfets.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                       _i32p, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.orc_solve_dc.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.POINTER(OrcDcOpts), _f64p,
+                                   C.POINTER(OrcDcOut)]
+        L.orc_solve_dc_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32,
+                                         C.POINTER(_f64p), C.POINTER(OrcDcOpts), C.c_int32, _f64p,
+                                         C.POINTER(OrcDcOut)]
+        _LIB = L
+    return _LIB
+
+
+def _p64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def _p32(a):
+    return a.ctypes.data_as(_i32p)
+
+
+class Topology:
+    """Oracle topology: pattern, stamp map, Min-Fill ordering, fill pattern."""
+
+    def __init__(self, nl):
+        self.nl = nl
+        self._c, self._keep = nl.to_c()
+        h = C.c_void_p()
+        rc = lib().orc_topology_build(C.byref(self._c), C.byref(h))
+        if rc:
+            raise RuntimeError(f"orc_topology_build failed: {rc}")
+        self.h = h
+        info = np.zeros(8, dtype=np.int32)
+        lib().orc_topology_info(h, _p32(info))
+        (self.n, self.nnz_a, self.nnz_f, self.n_stamps, self.treewidth,
+         self.ordering_len, self.fill_in, self.n_vars) = [int(x) for x in info]
+
+    def export(self, what):
+        cap = max(self.n + 1, self.nnz_a, self.nnz_f, self.n_stamps, self.ordering_len,
+                  len(self.nl.dio_is), len(self.nl.mos_w)) + 1
+        buf = np.zeros(cap, dtype=np.int32)
+        k = lib().orc_topology_export(self.h, what, _p32(buf), cap)
+        if k < 0:
+            raise RuntimeError("export failed")
+        return buf[:k].copy()
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_topology_free(self.h)
+            self.h = None
+
+
+def dc_opts(max_iter=50, gmin_steps=19, lu=LU_SPARSE, start_step=0, use_seed=0):
+    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed)
+
+
+def solve_dc(nl, topo: Topology | None = None, lu=LU_SPARSE, max_iter=50, gmin_steps=19,
+             seed_v=None, start_step=0):
+    c, keep = nl.to_c()
+    v = np.zeros(nl.num_nodes) if seed_v is None else np.array(seed_v, dtype=np.float64)
+    out = OrcDcOut()
+    o = dc_opts(max_iter, gmin_steps, lu, start_step, 0 if seed_v is None else 1)
+    lib().orc_solve_dc(C.byref(c), topo.h if topo else None, C.byref(o), _p64(v), C.byref(out))
+    return v, out
+
+
+def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_SPARSE, threads=0,
+                   max_iter=50, gmin_steps=19):
+    """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array."""
+    c, keep = nl.to_c()
+    arrs = (_f64p * _abi.N_FIELDS)()
+    hold = []
+    for fid in range(_abi.N_FIELDS):
+        if fid in fields:
+            a = np.ascontiguousarray(fields[fid][:B], dtype=np.float64)
+            hold.append(a)
+            arrs[fid] = _p64(a)
+        else:
+            arrs[fid] = None
+    v = np.zeros((B, nl.num_nodes))
+    outs = (OrcDcOut * B)()
+    o = dc_opts(max_iter, gmin_steps, lu)
+    lib().orc_solve_dc_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), outs)
+    stats = {
+        "iterations": np.array([x.iterations for x in outs]),
+        "ref_iterations": np.array([x.ref_iterations for x in outs]),
+        "status": np.array([x.status for x in outs]),
+        "kcl": np.array([x.kcl for x in outs]),
+        "gmin_steps_done": np.array([x.gmin_steps_done for x in outs]),
+    }
+    return v, stats
+
+
+def eval_diodes(a, c, Is, N, Vt, v):
+    a, c = np.ascontiguousarray(a, np.int32), np.ascontiguousarray(c, np.int32)
+    Is, N, Vt, v = (np.ascontiguousarray(x, np.float64) for x in (Is, N, Vt, v))
+    m = len(a)
+    vd, i_d, gd = np.zeros(m), np.zeros(m), np.zeros(m)
+    lib().orc_eval_diodes(m, _p32(a), _p32(c), _p64(Is), _p64(N), _p64(Vt), _p64(v), len(v),
+                          _p64(vd), _p64(i_d), _p64(gd))
+    return vd, i_d, gd
+
+
+def eval_mosfets(d, g, s, W, L, Kp, Vth, lam, pmos, v):
+    d, g, s, pmos = (np.ascontiguousarray(x, np.int32) for x in (d, g, s, pmos))
+    W, L, Kp, Vth, lam, v = (np.ascontiguousarray(x, np.float64) for x in (W, L, Kp, Vth, lam, v))
+    m = len(d)
+    outs = [np.zeros(m) for _ in range(5)]
+    lib().orc_eval_mosfets(m, _p32(d), _p32(g), _p32(s), _p64(W), _p64(L), _p64(Kp), _p64(Vth),
+                           _p64(lam), _p32(pmos), _p64(v), len(v), *[_p64(o) for o in outs])
+    return tuple(outs)
+
+
+def kcl(nl, v, per_node=False):
+    c, keep = nl.to_c()
+    v = np.ascontiguousarray(v, np.float64)
+    res = np.zeros(nl.num_nodes)
+    m = lib().orc_kcl(C.byref(c), None, _p64(v), _p64(res))
+    return (m, res) if per_node else m
+
+
+def stamp(nl, topo: Topology, v, gmin):
+    c, keep = nl.to_c()
+    v = np.ascontiguousarray(v, np.float64)
+    vals = np.zeros(topo.n_stamps)
+    rhs = np.zeros(nl.num_nodes)
+    rc = lib().orc_stamp(C.byref(c), topo.h, _p64(v), gmin, _p64(vals), _p64(rhs))
+    if rc:
+        raise RuntimeError("orc_stamp failed")
+    av = np.zeros(topo.nnz_a)
+    lib().orc_assemble(topo.h, _p64(vals), _p64(av))
+    return vals, av, rhs
+
+
+def lu_dense(n, rp, ci, vals, rhs):
+    x = np.zeros(n)
+    sp = C.c_double()
+    rc = lib().orc_lu_dense(n, _p32(np.ascontiguousarray(rp, np.int32)), _p32(np.ascontiguousarray(ci, np.int32)),
+                            _p64(np.ascontiguousarray(vals, np.float64)),
+                            _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(sp))
+    return rc, x
+
+
+def lu_sparse(topo: Topology, a_vals, rhs):
+    x = np.zeros(topo.n)
+    sp = C.c_double()
+    rc = lib().orc_lu_sparse(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
+                             _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(sp))
+    return rc, x
+
+
+def seed(nl):
+    c, keep = nl.to_c()
+    v = np.zeros(nl.num_nodes)
+    lib().orc_seed(C.byref(c), _p64(v))
+    return v
+
+
+def pnjlim(vnew, vold, vt, vcrit):
+    return lib().orc_pnjlim(vnew, vold, vt, vcrit)
+
+
+def vcrit(vt, Is):
+    return lib().orc_vcrit(vt, Is)

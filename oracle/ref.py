@@ -1,0 +1,161 @@
+"""ctypes binding of oracle/_ref/libkaram_ref.so — the reference engine's own
+translation units compiled from /root/reference by `make -C oracle ref`.
+
+TEST INFRASTRUCTURE ONLY.  Available only where the reference tree was
+present at build time; tests that need it skip when it is absent and rely on
+the committed fixtures under tests/golden/ instead.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from karam_amd import _abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PATH = os.path.join(_HERE, "_ref", "libkaram_ref.so")
+_LIB = None
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+
+
+def available() -> bool:
+    return os.path.exists(PATH)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        L = C.CDLL(PATH)
+        L.ref_tensorize.argtypes = [C.POINTER(_abi.NrNetlist)] + [_i32p, _i32p, _f64p, _f64p, _f64p,
+                                                                   _i32p, _i32p, _i32p, _i32p, _f64p, _f64p,
+                                                                   _f64p, _f64p, _f64p, _i32p]
+        L.ref_min_fill.argtypes = [C.POINTER(_abi.NrNetlist), _i32p, C.c_int64, _i32p, _i32p]
+        L.ref_pattern.argtypes = [C.c_int32, C.c_int32, _i32p, _i32p, _i32p, _i32p, C.c_int64, _i32p, _i32p]
+        L.ref_symbolic.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _i32p, _i32p, _i32p, C.c_int64, _i32p, _i32p]
+        L.ref_eval_diodes.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int32,
+                                      _f64p, _f64p, _f64p]
+        L.ref_eval_mosfets.argtypes = [C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                       _i32p, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.ref_lu_pivoted.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p]
+        for f in ("ref_pnjlim",):
+            getattr(L, f).argtypes = [C.c_double] * 4
+            getattr(L, f).restype = C.c_double
+        L.ref_vcrit.argtypes = [C.c_double] * 2
+        L.ref_vcrit.restype = C.c_double
+        L.ref_diode_i_clamped.argtypes = [C.c_double] * 4
+        L.ref_diode_i_clamped.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def _p32(a):
+    return a.ctypes.data_as(_i32p)
+
+
+def _p64(a):
+    return a.ctypes.data_as(_f64p)
+
+
+def tensorize(nl):
+    c, keep = nl.to_c()
+    nd, nm = len(nl.dio_is), len(nl.mos_w)
+    ia = {k: np.zeros(max(1, nd), np.int32) for k in ("d_a", "d_c")}
+    fa = {k: np.zeros(max(1, nd)) for k in ("d_is", "d_n", "d_vt")}
+    im = {k: np.zeros(max(1, nm), np.int32) for k in ("m_d", "m_g", "m_s", "m_b", "m_pmos")}
+    fm = {k: np.zeros(max(1, nm)) for k in ("m_w", "m_l", "m_kp", "m_vth", "m_lam")}
+    rc = lib().ref_tensorize(C.byref(c), _p32(ia["d_a"]), _p32(ia["d_c"]), _p64(fa["d_is"]), _p64(fa["d_n"]),
+                             _p64(fa["d_vt"]), _p32(im["m_d"]), _p32(im["m_g"]), _p32(im["m_s"]),
+                             _p32(im["m_b"]), _p64(fm["m_w"]), _p64(fm["m_l"]), _p64(fm["m_kp"]),
+                             _p64(fm["m_vth"]), _p64(fm["m_lam"]), _p32(im["m_pmos"]))
+    if rc:
+        raise RuntimeError("ref_tensorize failed")
+    out = {}
+    for k, v in {**ia, **fa}.items():
+        out[k] = v[:nd]
+    for k, v in {**im, **fm}.items():
+        out[k] = v[:nm]
+    return out
+
+
+def min_fill(nl):
+    c, keep = nl.to_c()
+    cap = nl.num_nodes + len(nl.vsrc_volts) + 8
+    ordering = np.zeros(cap, np.int32)
+    ln, tw = C.c_int32(), C.c_int32()
+    rc = lib().ref_min_fill(C.byref(c), _p32(ordering), cap, C.byref(ln), C.byref(tw))
+    if rc:
+        raise RuntimeError("ref_min_fill failed")
+    return ordering[: ln.value].copy(), tw.value
+
+
+def pattern(n, rows, cols):
+    rows = np.ascontiguousarray(rows, np.int32)
+    cols = np.ascontiguousarray(cols, np.int32)
+    rp = np.zeros(n + 1, np.int32)
+    ci = np.zeros(len(rows) + 1, np.int32)
+    nnz = C.c_int32()
+    slot = np.zeros(len(rows), np.int32)
+    rc = lib().ref_pattern(n, len(rows), _p32(rows), _p32(cols), _p32(rp), _p32(ci), ci.size, C.byref(nnz),
+                           _p32(slot))
+    if rc:
+        raise RuntimeError("ref_pattern failed")
+    return rp, ci[: nnz.value].copy(), slot
+
+
+def symbolic(n, rp, ci, ordering0):
+    rp = np.ascontiguousarray(rp, np.int32)
+    ci = np.ascontiguousarray(ci, np.int32)
+    ordering0 = np.ascontiguousarray(ordering0, np.int32)
+    perm = np.zeros(n, np.int32)
+    frp = np.zeros(n + 1, np.int32)
+    cap = n * n + 1 if n < 4096 else 4_000_000
+    fci = np.zeros(cap, np.int32)
+    nnzf, fill = C.c_int32(), C.c_int32()
+    rc = lib().ref_symbolic(n, _p32(rp), _p32(ci), _p32(ordering0), _p32(perm), _p32(frp), _p32(fci), cap,
+                            C.byref(nnzf), C.byref(fill))
+    if rc:
+        raise RuntimeError("ref_symbolic failed")
+    return perm, frp, fci[: nnzf.value].copy(), fill.value
+
+
+def eval_diodes(a, c, Is, N, Vt, v):
+    a, c = np.ascontiguousarray(a, np.int32), np.ascontiguousarray(c, np.int32)
+    Is, N, Vt, v = (np.ascontiguousarray(x, np.float64) for x in (Is, N, Vt, v))
+    m = len(a)
+    vd, i_d, gd = np.zeros(m), np.zeros(m), np.zeros(m)
+    lib().ref_eval_diodes(m, _p32(a), _p32(c), _p64(Is), _p64(N), _p64(Vt), _p64(v), len(v), _p64(vd),
+                          _p64(i_d), _p64(gd))
+    return vd, i_d, gd
+
+
+def eval_mosfets(d, g, s, W, L, Kp, Vth, lam, pmos, v):
+    d, g, s, pmos = (np.ascontiguousarray(x, np.int32) for x in (d, g, s, pmos))
+    W, L, Kp, Vth, lam, v = (np.ascontiguousarray(x, np.float64) for x in (W, L, Kp, Vth, lam, v))
+    m = len(d)
+    outs = [np.zeros(m) for _ in range(5)]
+    lib().ref_eval_mosfets(m, _p32(d), _p32(g), _p32(s), _p64(W), _p64(L), _p64(Kp), _p64(Vth), _p64(lam),
+                           _p32(pmos), _p64(v), len(v), *[_p64(o) for o in outs])
+    return tuple(outs)
+
+
+def lu_pivoted(n, rp, ci, vals, rhs):
+    x = np.zeros(n)
+    rc = lib().ref_lu_pivoted(n, _p32(np.ascontiguousarray(rp, np.int32)), _p32(np.ascontiguousarray(ci, np.int32)),
+                              _p64(np.ascontiguousarray(vals, np.float64)),
+                              _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x))
+    return rc, x
+
+
+def pnjlim(vnew, vold, vt, vcrit):
+    return lib().ref_pnjlim(vnew, vold, vt, vcrit)
+
+
+def vcrit(vt, Is):
+    return lib().ref_vcrit(vt, Is)
+
+
+def diode_i_clamped(vd, Is, N, Vt):
+    return lib().ref_diode_i_clamped(vd, Is, N, Vt)

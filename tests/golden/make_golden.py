@@ -1,0 +1,185 @@
+"""Generate the golden fixtures under tests/golden/ from the reference's own
+translation units (oracle/_ref/libkaram_ref.so, built by
+`make -C oracle ref` from /root/reference).
+
+Each fixture stores the netlist arrays it was made from (inputs) and the
+reference outputs: tensorizeNetlist SoA arrays, the Min-Fill ordering and
+treewidth, the CSR pattern and stamp map of MatrixConstructor::buildPattern,
+symbolicFactorize's permutation / fill pattern / fill count, generic device
+kernel outputs on voltage grids, pnjlim / compute_vcrit / diode_model_pair
+values and a solveLU_Pivoted solution.  Stamp coordinates fed to
+buildPattern are the oracle's emission order (stampSoABlock itself lives in
+a header that needs sources absent from the reference tree).
+
+Run from the repository root:  python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, ROOT)
+
+from karam_amd import netlists  # noqa: E402
+from karam_amd.netlists import ModelCard, Netlist, _f64, _i32  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from oracle import ref as R  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def mixed_netlist(seed: int) -> Netlist:
+    """Small random netlist exercising every tensorizer / stamp branch:
+    floating diodes, several diode models, MOSFET cards with missing
+    parameters, a card-less model name containing PMOS, floating and
+    grounded sources, bodies on non-ground nodes."""
+    rng = np.random.default_rng(seed)
+    n = 14
+    nR = 24
+    r1 = rng.integers(0, n + 1, nR)
+    r2 = (r1 + rng.integers(1, n, nR)) % (n + 1)
+    R = rng.uniform(100, 10e3, nR)
+    dio_a = rng.integers(0, n + 1, 6)
+    dio_c = rng.integers(0, n + 1, 6)
+    dio_c[dio_c == dio_a] = 0
+    dio_a[(dio_a == 0) & (dio_c == 0)] = 3
+    models = [
+        ModelCard("nch", "NMOS", kp=150e-6, vto=0.6, lam=None),
+        ModelCard("pch", "PMOS", kp=None, vto=-0.65, lam=0.03),
+        ModelCard("weird_pmos_x", None),
+        ModelCard("Dfast", None),
+        ModelCard("Aslow", None),
+    ]
+    nm = 8
+    md, mg, ms, mb = (rng.integers(0, n + 1, nm) for _ in range(4))
+    mm = rng.choice([0, 1, 2], nm)
+    return Netlist(
+        num_nodes=n,
+        res_n1=_i32(r1), res_n2=_i32(r2), res_ohms=_f64(R),
+        vsrc_pos=_i32([1, 5]), vsrc_neg=_i32([0, 6]), vsrc_volts=_f64([1.8, 0.3]),
+        isrc_pos=_i32([0, 7]), isrc_neg=_i32([9, 0]), isrc_amps=_f64([1e-3, 2e-4]),
+        dio_anode=_i32(dio_a), dio_cathode=_i32(dio_c), dio_is=_f64(10 ** rng.uniform(-15, -12, 6)),
+        dio_n=_f64(rng.uniform(1, 1.5, 6)), dio_vt=_f64(np.full(6, 0.02585)),
+        dio_model=_i32([3, -1, 4, 3, -1, 4]),
+        mos_d=_i32(md), mos_g=_i32(mg), mos_s=_i32(ms), mos_b=_i32(mb),
+        mos_w=_f64(rng.uniform(1e-6, 5e-6, nm)), mos_l=_f64(np.full(nm, 1e-6)), mos_model=_i32(mm),
+        models=models, name=f"mixed{seed}",
+    )
+
+
+def netlist_arrays(nl: Netlist) -> dict:
+    out = {"num_nodes": np.int32(nl.num_nodes), "ambient_temp_K": np.float64(nl.ambient_temp_K)}
+    for k in ("res_n1", "res_n2", "res_ohms", "cap_n1", "cap_n2", "cap_farads", "vsrc_pos", "vsrc_neg",
+              "vsrc_volts", "isrc_pos", "isrc_neg", "isrc_amps", "dio_anode", "dio_cathode", "dio_is", "dio_n",
+              "dio_vt", "mos_d", "mos_g", "mos_s", "mos_b", "mos_w", "mos_l", "mos_model"):
+        out["nl_" + k] = np.asarray(getattr(nl, k))
+    out["nl_dio_model"] = np.asarray(nl.dio_model if nl.dio_model is not None else np.full(len(nl.dio_is), -1),
+                                     np.int32)
+    out["nl_models"] = np.array([f"{m.name}|{m.type or ''}|{'' if m.kp is None else repr(m.kp)}|"
+                                 f"{'' if m.vto is None else repr(m.vto)}|{'' if m.lam is None else repr(m.lam)}"
+                                 for m in nl.models], dtype="U128")
+    return out
+
+
+def topology_fixture(nl: Netlist) -> dict:
+    T = O.Topology(nl)
+    out = netlist_arrays(nl)
+    soa = R.tensorize(nl)
+    for k, v in soa.items():
+        out["ref_" + k] = v
+    ordering, tw = R.min_fill(nl)
+    out["ref_ordering_raw"] = ordering
+    out["ref_treewidth"] = np.int32(tw)
+    rows, cols = T.export(100), T.export(101)
+    out["stamp_rows"], out["stamp_cols"] = rows, cols
+    rp, ci, slot = R.pattern(nl.num_nodes, rows, cols)
+    out["ref_row_ptr"], out["ref_col_idx"], out["ref_stamp_slot"] = rp, ci, slot
+    # D3/D4 projection of the reference ordering onto rows
+    seen, perm0 = set(), []
+    for i in ordering:
+        if 1 <= i <= nl.num_nodes and i not in seen:
+            seen.add(int(i))
+            perm0.append(int(i) - 1)
+    perm0 += [i - 1 for i in range(1, nl.num_nodes + 1) if i not in seen]
+    perm, frp, fci, fill = R.symbolic(nl.num_nodes, rp, ci, np.array(perm0, np.int32))
+    out["ref_perm"], out["ref_frow_ptr"], out["ref_fcol_idx"], out["ref_fill_in"] = perm, frp, fci, np.int32(fill)
+    # a solveLU_Pivoted solution of the first NR system (gmin 1e-3 at the seed)
+    v0 = O.seed(nl)
+    vals, av, rhs = O.stamp(nl, T, v0, 1e-3)
+    rc, x = R.lu_pivoted(nl.num_nodes, rp, ci, av, rhs)
+    out["lu_v0"], out["lu_a_vals"], out["lu_rhs"], out["ref_lu_x"], out["ref_lu_rc"] = v0, av, rhs, x, np.int32(rc)
+    return out
+
+
+def device_fixture() -> dict:
+    rng = np.random.default_rng(7)
+    out = {}
+    # diodes: voltages around 0, the 30 N Vt continuation knee and reverse bias
+    nv = 40
+    v = np.concatenate([rng.uniform(-3, 3, nv - 8), [0.0, 0.7755, 0.7756, 1.2, -0.5, 30 * 0.02585, 1e-9, 2.0]])
+    m = 400
+    a = rng.integers(0, nv + 1, m)
+    c = rng.integers(0, nv + 1, m)
+    Is = 10 ** rng.uniform(-16, -9, m)
+    N = rng.choice([1.0, 1.2, 2.0], m)
+    Vt = rng.choice([0.02585, 0.0259], m)
+    vd, i_d, gd = R.eval_diodes(a, c, Is, N, Vt, v)
+    out.update(dio_v=v, dio_a=a.astype(np.int32), dio_c=c.astype(np.int32), dio_Is=Is, dio_N=N, dio_Vt=Vt,
+               ref_dio_vd=vd, ref_dio_id=i_d, ref_dio_gd=gd)
+    # MOSFETs: include the region boundaries vov = 0 and vds = vov exactly
+    vm = np.concatenate([rng.uniform(-2, 2, 36), [0.0, 0.7, 1.4, -0.7]])
+    mm = 600
+    d, g, s = (rng.integers(0, len(vm) + 1, mm) for _ in range(3))
+    W = rng.uniform(1e-6, 8e-6, mm)
+    L = rng.choice([0.5e-6, 1e-6], mm)
+    Kp = rng.uniform(20e-6, 300e-6, mm)
+    Vth = rng.choice([0.7, -0.7, 0.0, 0.5], mm)
+    lam = rng.choice([0.0, 0.02, 0.05], mm)
+    pm = rng.integers(0, 2, mm)
+    res = R.eval_mosfets(d, g, s, W, L, Kp, Vth, lam, pm, vm)
+    out.update(mos_v=vm, mos_d=d.astype(np.int32), mos_g=g.astype(np.int32), mos_s=s.astype(np.int32), mos_W=W,
+               mos_L=L, mos_Kp=Kp, mos_Vth=Vth, mos_lam=lam, mos_pm=pm.astype(np.int32))
+    for k, x in zip(("vgs", "vds", "ids", "gm", "gds"), res):
+        out["ref_mos_" + k] = x
+    # pnjlim / vcrit / diode_model_pair
+    k = 500
+    vt = rng.choice([0.02585, 0.03], k)
+    isv = 10 ** rng.uniform(-16, -10, k)
+    vcrit = np.array([R.vcrit(a_, b_) for a_, b_ in zip(vt, isv)])
+    vnew = rng.uniform(-2, 2, k)
+    vold = rng.uniform(-2, 2, k)
+    vold[:50] = 0.0
+    vold[50:100] = rng.uniform(0.8, 1.0, 50)
+    vnew[50:100] = vold[50:100] + rng.uniform(-0.2, 0.2, 50)
+    lim = np.array([R.pnjlim(a_, b_, c_, d_) for a_, b_, c_, d_ in zip(vnew, vold, vt, vcrit)])
+    out.update(pn_vt=vt, pn_is=isv, pn_vnew=vnew, pn_vold=vold, ref_pn_vcrit=vcrit, ref_pn_lim=lim)
+    vd2 = rng.uniform(-1, 1.5, k)
+    nn = rng.choice([1.0, 1.5], k)
+    out.update(dc_vd=vd2, dc_is=isv, dc_n=nn, dc_vt=vt,
+               ref_dc_i=np.array([R.diode_i_clamped(a_, b_, c_, d_) for a_, b_, c_, d_ in zip(vd2, isv, nn, vt)]))
+    return out
+
+
+def main():
+    if not R.available():
+        raise SystemExit("oracle/_ref/libkaram_ref.so missing: run `make -C oracle ref` first")
+    cases = {
+        "inv16": netlists.inverter_chain(),
+        "rdmesh8": netlists.rd_mesh(8, seed=1),
+        "rdmesh16": netlists.rd_mesh(16, seed=2),
+        "mixed1": mixed_netlist(1),
+        "mixed2": mixed_netlist(2),
+    }
+    for name, nl in cases.items():
+        fx = topology_fixture(nl)
+        np.savez_compressed(os.path.join(OUT, f"topo_{name}.npz"), **fx)
+        print("wrote", name, "n", nl.num_nodes, "ordering", len(fx["ref_ordering_raw"]), "tw", int(fx["ref_treewidth"]))
+    np.savez_compressed(os.path.join(OUT, "device_eval.npz"), **device_fixture())
+    print("wrote device_eval")
+
+
+if __name__ == "__main__":
+    main()

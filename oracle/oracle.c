@@ -245,6 +245,11 @@ struct orc_topo {
     int32_t *perm, *iperm;
     int32_t nnz_f, fill_in;
     int32_t *frow_ptr, *fcol_idx, *fdiag;
+    /* fundamental supernodes of the fill pattern (multifrontal LU) */
+    int32_t ns;
+    int32_t *sn_first, *sn_ncols, *sn_nrows, *sn_row_off, *sn_rows;
+    int32_t *sn_child_ptr, *sn_child;
+    int32_t *sn_a_ptr, *sn_a_slot, *sn_a_li, *sn_a_lj;
 };
 
 /* growable int vector */
@@ -573,6 +578,90 @@ static void symbolic(orc_topo* t) {
     free(rows);
 }
 
+/* Fundamental supernodes of the filled pattern: column j-1 joins column j
+ * when parent(j-1) = j, |struct L(:,j-1)| = |struct L(:,j)| + 1 and j has no
+ * other child.  Children are listed in increasing supernode index; the
+ * front of supernode J is ordered [own columns, then its update rows R]. */
+static int32_t front_local(const orc_topo* t, int32_t J, int32_t x) {
+    int32_t f = t->sn_first[J], c = t->sn_ncols[J];
+    if (x >= f && x < f + c) return x - f;
+    const int32_t* R = t->sn_rows + t->sn_row_off[J];
+    int32_t lo = 0, hi = t->sn_nrows[J];
+    while (lo < hi) { int32_t mid = (lo + hi) / 2; if (R[mid] < x) lo = mid + 1; else hi = mid; }
+    return (lo < t->sn_nrows[J] && R[lo] == x) ? c + lo : -1;
+}
+
+static void supernodes(orc_topo* t) {
+    const int32_t n = t->n;
+    int32_t* cnt = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)n);     /* |struct L(:,j)| off-diagonal */
+    int32_t* par = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)n);
+    int32_t* nch = (int32_t*)xcalloc((size_t)n, 4);
+    for (int32_t j = 0; j < n; ++j) {
+        cnt[j] = t->frow_ptr[j + 1] - t->fdiag[j] - 1;
+        par[j] = cnt[j] > 0 ? t->fcol_idx[t->fdiag[j] + 1] : -1;
+        if (par[j] >= 0) nch[par[j]]++;
+    }
+    int32_t* start = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    int32_t ns = 0;
+    start[ns++] = 0;
+    for (int32_t j = 1; j < n; ++j)
+        if (!(par[j - 1] == j && cnt[j - 1] == cnt[j] + 1 && nch[j] == 1)) start[ns++] = j;
+    start[ns] = n;
+    t->ns = ns;
+    t->sn_first = (int32_t*)xmalloc(4 * (size_t)ns); t->sn_ncols = (int32_t*)xmalloc(4 * (size_t)ns);
+    t->sn_nrows = (int32_t*)xmalloc(4 * (size_t)ns); t->sn_row_off = (int32_t*)xmalloc(4 * (size_t)ns);
+    int32_t* col2sn = (int32_t*)xmalloc(4 * (size_t)n);
+    int64_t rtot = 0;
+    for (int32_t J = 0; J < ns; ++J) {
+        t->sn_first[J] = start[J];
+        t->sn_ncols[J] = start[J + 1] - start[J];
+        t->sn_nrows[J] = cnt[start[J + 1] - 1];
+        t->sn_row_off[J] = (int32_t)rtot;
+        rtot += t->sn_nrows[J];
+        for (int32_t j = start[J]; j < start[J + 1]; ++j) col2sn[j] = J;
+    }
+    t->sn_rows = (int32_t*)xmalloc(4 * (size_t)(rtot + 1));
+    for (int32_t J = 0; J < ns; ++J) {
+        int32_t last = start[J + 1] - 1;
+        memcpy(t->sn_rows + t->sn_row_off[J], t->fcol_idx + t->fdiag[last] + 1, 4 * (size_t)t->sn_nrows[J]);
+    }
+    /* children in increasing index */
+    t->sn_child_ptr = (int32_t*)xcalloc((size_t)ns + 1, 4);
+    int32_t* snpar = (int32_t*)xmalloc(4 * (size_t)ns);
+    for (int32_t J = 0; J < ns; ++J) {
+        int32_t last = start[J + 1] - 1;
+        snpar[J] = par[last] >= 0 ? col2sn[par[last]] : -1;
+        if (snpar[J] >= 0) t->sn_child_ptr[snpar[J] + 1]++;
+    }
+    for (int32_t J = 0; J < ns; ++J) t->sn_child_ptr[J + 1] += t->sn_child_ptr[J];
+    t->sn_child = (int32_t*)xmalloc(4 * (size_t)(ns + 1));
+    int32_t* fillp = (int32_t*)xmalloc(4 * (size_t)(ns + 1));
+    memcpy(fillp, t->sn_child_ptr, 4 * (size_t)ns);
+    for (int32_t J = 0; J < ns; ++J) if (snpar[J] >= 0) t->sn_child[fillp[snpar[J]]++] = J;
+    /* A entries per supernode (CSR order) */
+    t->sn_a_ptr = (int32_t*)xcalloc((size_t)ns + 1, 4);
+    t->sn_a_slot = (int32_t*)xmalloc(4 * (size_t)(t->nnz_a + 1));
+    t->sn_a_li = (int32_t*)xmalloc(4 * (size_t)(t->nnz_a + 1));
+    t->sn_a_lj = (int32_t*)xmalloc(4 * (size_t)(t->nnz_a + 1));
+    int32_t* owner = (int32_t*)xmalloc(4 * (size_t)(t->nnz_a + 1));
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t k = t->row_ptr[r]; k < t->row_ptr[r + 1]; ++k) {
+            int32_t pi = t->iperm[r], pj = t->iperm[t->col_idx[k]];
+            owner[k] = col2sn[pi < pj ? pi : pj];
+            t->sn_a_ptr[owner[k] + 1]++;
+        }
+    for (int32_t J = 0; J < ns; ++J) t->sn_a_ptr[J + 1] += t->sn_a_ptr[J];
+    memcpy(fillp, t->sn_a_ptr, 4 * (size_t)ns);
+    for (int32_t r = 0; r < n; ++r)
+        for (int32_t k = t->row_ptr[r]; k < t->row_ptr[r + 1]; ++k) {
+            int32_t J = owner[k], q = fillp[J]++;
+            t->sn_a_slot[q] = k;
+            t->sn_a_li[q] = front_local(t, J, t->iperm[r]);
+            t->sn_a_lj[q] = front_local(t, J, t->iperm[t->col_idx[k]]);
+        }
+    free(owner); free(fillp); free(snpar); free(col2sn); free(start); free(cnt); free(par); free(nch);
+}
+
 /* a8: createCsr / buildPattern (math/linalg.h:281-431): sort the triplet
  * coordinates by (row, col), merge duplicates; the stamp map is each
  * triplet's merged slot (assembleFast's position_map lookup, :441-465). */
@@ -622,6 +711,7 @@ int orc_topology_build(const nr_netlist* nl, orc_topo** out) {
     min_fill(nl, t->soa, t);
     project_ordering(t);
     symbolic(t);
+    supernodes(t);
     *out = t;
     return NR_OK;
 }
@@ -632,6 +722,9 @@ void orc_topology_free(orc_topo* t) {
     free(t->st_row); free(t->st_col); free(t->row_ptr); free(t->col_idx); free(t->stamp_slot);
     free(t->ordering_raw); free(t->perm); free(t->iperm);
     free(t->frow_ptr); free(t->fcol_idx); free(t->fdiag);
+    free(t->sn_first); free(t->sn_ncols); free(t->sn_nrows); free(t->sn_row_off); free(t->sn_rows);
+    free(t->sn_child_ptr); free(t->sn_child);
+    free(t->sn_a_ptr); free(t->sn_a_slot); free(t->sn_a_li); free(t->sn_a_lj);
     free(t);
 }
 
@@ -805,6 +898,91 @@ int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
     return rc;
 }
 
+int orc_lu_multifrontal(const orc_topo* t, const double* a_vals, const double* rhs,
+                        double* x, double* smallest_pivot) {
+    const int32_t n = t->n, ns = t->ns;
+    int64_t* u_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* cb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* rcb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t ut = 0, ct = 0, rt = 0;
+    int32_t rmax = 1;
+    for (int32_t J = 0; J < ns; ++J) {
+        int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m;
+        u_off[J] = ut; ut += (int64_t)c * r;
+        cb_off[J] = ct; ct += (int64_t)m * m;
+        rcb_off[J] = rt; rt += m;
+        if (r > rmax) rmax = r;
+    }
+    double* U = (double*)xmalloc(8 * (size_t)(ut + 1));
+    double* CB = (double*)xmalloc(8 * (size_t)(ct + 1));
+    double* RCB = (double*)xmalloc(8 * (size_t)(rt + 1));
+    double* y = (double*)xmalloc(8 * (size_t)n);
+    double* xp = (double*)xmalloc(8 * (size_t)n);
+    double* F = (double*)xmalloc(8 * (size_t)rmax * (size_t)rmax);
+    double* fr = (double*)xmalloc(8 * (size_t)rmax);
+    double smallest = 1e300;
+    int rc = NR_OK;
+    for (int32_t J = 0; J < ns; ++J) {
+        const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+        for (int32_t e = 0; e < r * r; ++e) F[e] = 0.0;
+        for (int32_t e = 0; e < r; ++e) fr[e] = 0.0;
+        for (int32_t q = t->sn_a_ptr[J]; q < t->sn_a_ptr[J + 1]; ++q)
+            F[t->sn_a_li[q] * r + t->sn_a_lj[q]] += a_vals[t->sn_a_slot[q]];
+        for (int32_t k = 0; k < c; ++k) fr[k] += rhs[t->perm[f + k]];
+        for (int32_t q = t->sn_child_ptr[J]; q < t->sn_child_ptr[J + 1]; ++q) {
+            const int32_t C = t->sn_child[q], mc = t->sn_nrows[C];
+            const int32_t* Rc = t->sn_rows + t->sn_row_off[C];
+            int32_t* rel = (int32_t*)xmalloc(4 * (size_t)(mc + 1));
+            for (int32_t a = 0; a < mc; ++a) rel[a] = front_local(t, J, Rc[a]);
+            const double* cb = CB + cb_off[C];
+            for (int32_t a = 0; a < mc; ++a)
+                for (int32_t b = 0; b < mc; ++b) F[rel[a] * r + rel[b]] += cb[a * mc + b];
+            for (int32_t a = 0; a < mc; ++a) fr[rel[a]] += RCB[rcb_off[C] + a];
+            free(rel);
+        }
+        for (int32_t k = 0; k < c; ++k) {
+            const double piv = F[k * r + k];
+            if (fabs(piv) < smallest) smallest = fabs(piv);
+            if (!(fabs(piv) >= 1e-25)) rc = NR_STATUS_SINGULAR;
+            for (int32_t i = k + 1; i < r; ++i) F[i * r + k] = F[i * r + k] / piv;
+            for (int32_t i = k + 1; i < r; ++i) {
+                const double l = F[i * r + k];
+                for (int32_t j = k + 1; j < r; ++j) F[i * r + j] = fma(-l, F[k * r + j], F[i * r + j]);
+                fr[i] = fma(-l, fr[k], fr[i]);
+            }
+        }
+        memcpy(U + u_off[J], F, 8 * (size_t)c * (size_t)r);
+        for (int32_t k = 0; k < c; ++k) y[f + k] = fr[k];
+        for (int32_t a = 0; a < m; ++a) {
+            for (int32_t b = 0; b < m; ++b) CB[cb_off[J] + a * m + b] = F[(c + a) * r + c + b];
+            RCB[rcb_off[J] + a] = fr[c + a];
+        }
+    }
+    if (rc == NR_OK) {
+        for (int32_t J = ns - 1; J >= 0; --J) {
+            const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+            const double* Uj = U + u_off[J];
+            const int32_t* R = t->sn_rows + t->sn_row_off[J];
+            for (int32_t k = 0; k < c; ++k) {
+                double s = y[f + k];
+                for (int32_t a = 0; a < m; ++a) s = fma(-Uj[k * r + c + a], xp[R[a]], s);
+                fr[k] = s;
+            }
+            for (int32_t k = c - 1; k >= 0; --k) {
+                const double xk = fr[k] / Uj[k * r + k];
+                xp[f + k] = xk;
+                for (int32_t i = 0; i < k; ++i) fr[i] = fma(-Uj[i * r + k], xk, fr[i]);
+            }
+        }
+        for (int32_t i = 0; i < n; ++i) x[t->perm[i]] = xp[i];
+    } else {
+        for (int32_t i = 0; i < n; ++i) x[i] = 0.0;
+    }
+    if (smallest_pivot) *smallest_pivot = smallest;
+    free(u_off); free(cb_off); free(rcb_off); free(U); free(CB); free(RCB); free(y); free(xp); free(F); free(fr);
+    return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* a14: seedInitialGuess (solvers/circuitsim_solver.cpp:32-116)             */
 /* ------------------------------------------------------------------------ */
@@ -970,7 +1148,7 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 /* a15: solveDC (solvers/circuitsim_solver.cpp:176-555)                     */
 /* ------------------------------------------------------------------------ */
 void orc_dc_opts_default(orc_dc_opts* o) {
-    o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_SPARSE;
+    o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_MULTIFRONTAL;
     o->start_step = 0; o->use_seed = 0;
 }
 
@@ -1005,7 +1183,8 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
             orc_assemble(t, vals, av);
             int rc = (o.lu_kind == ORC_LU_DENSE)
                          ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
-                         : orc_lu_sparse(t, av, rhs, vn, NULL);
+                         : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
+                                                        : orc_lu_multifrontal(t, av, rhs, vn, NULL);
             if (rc != NR_OK) {
                 final_kcl = 1e9;
                 singular = 1; sing_step = 1;

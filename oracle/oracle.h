@@ -64,6 +64,12 @@ int orc_lu_dense(int32_t n, const int32_t* rp, const int32_t* ci,
 /* Same-pattern sparse LU on the symbolic fill pattern, no pivoting. */
 int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
                   double* x, double* smallest_pivot);
+/* Multifrontal LU on the fundamental supernodes of the same fill pattern,
+ * no pivoting, with one fixed operation order (assembly: A entry, then the
+ * children's update blocks in child order; elimination: explicit FMAs).
+ * The HIP library performs exactly these operations in this order. */
+int orc_lu_multifrontal(const orc_topo* t, const double* a_vals, const double* rhs,
+                        double* x, double* smallest_pivot);
 
 /* --- a14, a18, a19, a20 ---------------------------------------------------- */
 void orc_seed(const nr_netlist* nl, double* v);
@@ -77,7 +83,7 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
                           double kcl, double kcl_tol);
 
 /* --- a15: solveDC ---------------------------------------------------------- */
-enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1 };
+enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2 };
 typedef struct orc_dc_opts {
     int32_t max_iter;     /* 50 */
     int32_t gmin_steps;   /* 19 */

@@ -33,6 +33,7 @@ class OrcDcOut(C.Structure):
 
 LU_DENSE = 0
 LU_SPARSE = 1
+LU_MULTIFRONTAL = 2
 
 
 def lib():
@@ -112,11 +113,11 @@ class Topology:
             self.h = None
 
 
-def dc_opts(max_iter=50, gmin_steps=19, lu=LU_SPARSE, start_step=0, use_seed=0):
+def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0):
     return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed)
 
 
-def solve_dc(nl, topo: Topology | None = None, lu=LU_SPARSE, max_iter=50, gmin_steps=19,
+def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, gmin_steps=19,
              seed_v=None, start_step=0):
     c, keep = nl.to_c()
     v = np.zeros(nl.num_nodes) if seed_v is None else np.array(seed_v, dtype=np.float64)
@@ -126,7 +127,7 @@ def solve_dc(nl, topo: Topology | None = None, lu=LU_SPARSE, max_iter=50, gmin_s
     return v, out
 
 
-def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_SPARSE, threads=0,
+def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL, threads=0,
                    max_iter=50, gmin_steps=19):
     """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array."""
     c, keep = nl.to_c()

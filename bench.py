@@ -1,0 +1,192 @@
+"""Benchmark: Monte-Carlo DC operating points of the 64x64 resistor-diode mesh.
+
+One step = one batched DC solve of all instances to convergence (19-step
+GMIN plan, solveDC semantics), with netlist parameters already resident in
+HBM.  Prints ONE JSON line:
+
+  value        converged solves/s (FP64 KCL <= 1e-9 A), whole job, 1 GPU
+  roofline     the kernel with the largest device time in the timed steps:
+               algorithmic bytes per launch / its mean HIP-event launch time
+  cpu_baseline the oracle (FP64 C restatement, OpenMP over instances) timed
+               on this host's cores over a bounded sample of the instances,
+               whose voltages are also compared node by node with the GPU's
+
+    python bench.py [--steps K] [--warmup W] [--instances B] [--cpu-seconds S]
+    python bench.py --copy-probe      # plain device copy for counter scaling
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MC circuit solves/sec at KCL resid ≤1e-9 A; NR iters/solve, seeded vs plain"
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, public spec
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_rdmesh64_b4096.md")
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def copy_probe(gib=2):
+    """A plain device-to-device copy of a known size, for scaling the
+    FETCH_SIZE / WRITE_SIZE counters into HBM bytes."""
+    import torch
+    n = gib * (1 << 30) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}), flush=True)
+
+
+def load_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (lines `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    pat = re.compile(r"^traffic_bytes_per_launch\s+(\S+)\s+([0-9.eE+]+)")
+    for ln in open(TRAFFIC_FILE):
+        m = pat.match(ln.strip().strip("`"))
+        if m and m.group(1) == kernel:
+            return float(m.group(2))
+    return None
+
+
+def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
+    """Oracle over a bounded sample of the same instances on `threads` host
+    threads; returns (baseline dict, parity dict)."""
+    from oracle import oracle as O
+    T = O.Topology(nl)
+    n0 = min(B, threads)
+    t = time.perf_counter()
+    Vo, so = O.solve_dc_batch(nl, T, {k: v[:n0] for k, v in fields.items()}, n0, threads=threads)
+    dt0 = time.perf_counter() - t
+    n = int(min(B, max(n0, (seconds / max(dt0, 1e-3)) * n0) // threads * threads))
+    n = max(n, n0)
+    if n > n0:
+        t = time.perf_counter()
+        Vr, sr = O.solve_dc_batch(nl, T, {k: v[n0:n] for k, v in fields.items()}, n - n0, threads=threads)
+        dt1 = time.perf_counter() - t
+        Vo = np.concatenate([Vo, Vr])
+        so = {k: np.concatenate([so[k], sr[k]]) for k in so}
+        rate = (n - n0) / dt1
+    else:
+        rate = n0 / dt0
+    err = np.abs(V_gpu[:n] - Vo) / (1e-9 * np.abs(Vo) + 1e-12)
+    base = {"value": rate, "unit": "solves/s", "cores": threads, "kind": "port",
+            "sample": f"first {n} of {B} instances (same netlist and parameters), FP64 oracle "
+                      f"(multifrontal LU on the reference fill pattern), OpenMP {threads} threads"}
+    parity = {"instances": n, "max_err_over_tol": float(err.max()),
+              "within_tol": int((err.max(axis=1) <= 1.0).sum()),
+              "iterations_equal": int((it_gpu[:n] == so["iterations"]).sum()),
+              "tolerance": "1e-9 rel + 1e-12 V abs"}
+    return base, parity
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--side", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
+    a = ap.parse_args()
+    if a.copy_probe:
+        return copy_probe()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: Monte-Carlo instances shard by contiguous ranges with no exchange, "
+              "and multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
+        return 2
+
+    from karam_amd import _abi, netlists, nr
+
+    B = a.instances
+    nl = netlists.rd_mesh(a.side, seed=1)
+    fields = netlists.mc_fields(nl, B, seed=1000)
+    ctx = nr.Context(0)
+    t = time.perf_counter()
+    T = nr.Topology(nl, ctx)
+    symbolic_ms = (time.perf_counter() - t) * 1e3
+    bt = nr.Batch(ctx, T, B)
+    bt.set_fields(fields)
+    log(f"workload rdmesh{a.side} B={B} n={T.info['n']} nnz_f={T.info['nnz_f']} symbolic {symbolic_ms:.1f} ms")
+    for w in range(a.warmup):
+        r = bt.solve_dc()
+        log(f"warmup {w}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
+
+    kst = {}
+    conv = 0
+    sweeps = []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        r = bt.solve_dc(record_timing=2)
+        st = bt.stats()
+        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+        sweeps.append(r["batched_iterations"])
+        for name, x in bt.kernel_stats().items():
+            acc = kst.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            for f in acc:
+                acc[f] += x[f]
+        log(f"step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
+    total = time.perf_counter() - t0
+    V = bt.voltages()
+    st = bt.stats()
+    evals = r["device_evals"] * a.steps
+
+    dom = max(kst, key=lambda n: kst[n]["ms"])
+    d = kst[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    traffic = load_traffic(dom)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic, "kernel": dom,
+            "algorithmic_bytes_per_launch": d["bytes"] / d["launches"],
+            "mean_launch_us": d["ms"] * 1e3 / d["launches"], "launches": d["launches"],
+            "device_ms_share": d["ms"] / sum(x["ms"] for x in kst.values())}
+    it = st["iterations"]
+    out = {
+        "metric": METRIC, "value": conv / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": total * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded 64x64 R-diode mesh, per-instance Gaussian sigma 5% on every R and IS",
+        "config": {"workload": f"rdmesh{a.side} Monte-Carlo DC operating point, plain FP64 batched GPU path",
+                   "instances": B, "nodes": T.info["n"], "resistors": len(nl.res_ohms),
+                   "diodes": len(nl.dio_is), "current_sources": len(nl.isrc_amps),
+                   "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
+                   "gmin_steps": 19, "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": sweeps[-1]},
+        "nr_iters_per_solve": {"plain": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
+                                         "p99": float(np.percentile(it, 99))}, "seeded": None},
+        "device_evals_per_s": evals / total,
+        "roofline": roof,
+        "kernels": {n: {"launches": x["launches"], "ms": round(x["ms"], 3),
+                        "GBps": round(x["bytes"] / max(x["ms"], 1e-9) / 1e6, 1)} for n, x in
+                    sorted(kst.items(), key=lambda kv: -kv[1]["ms"])},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        base, parity = cpu_baseline(nl, fields, B, V, it, a.cpu_seconds, threads)
+        out["cpu_baseline"] = base
+        out["parity_sample"] = parity
+        log(f"cpu baseline {base['value']:.1f} solves/s on {threads} threads; parity {parity}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

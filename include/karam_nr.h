@@ -202,7 +202,8 @@ typedef struct nr_solve_opts {
     int32_t gmin_steps;      /* reference GMIN_STEPS = 19                     */
     int32_t precision;       /* NR_PREC_*                                     */
     int32_t seeded_start_step; /* GMIN step where seeded instances start      */
-    int32_t record_timing;   /* 1: per-phase HIP event timing                 */
+    int32_t record_timing;   /* 1: per-phase HIP event timing; 2: also per
+                                kernel launch (nr_batch_get_kernel_stats)    */
     int32_t use_graph;       /* 1: replay each batched iteration as a hipGraph*/
 } nr_solve_opts;
 NR_API void nr_solve_opts_default(nr_solve_opts* opts);
@@ -257,6 +258,25 @@ typedef struct nr_timing {
     int64_t factor_instances;    /* instance-factorisations performed         */
 } nr_timing;
 NR_API int nr_batch_get_timing(const nr_batch* batch, nr_timing* out);
+
+/* Per-kernel accounting of the last solve run with record_timing = 2: one
+ * row per kernel symbol, with its launch count, the sum of its per-launch
+ * HIP event durations (recorded on the batch's stream around each launch),
+ * and the algorithmic HBM bytes and FP64 flops of those launches (sizes of
+ * the work itself, independent of how the kernel moves data).  No
+ * counterpart in the reference, which has no device kernels on this path;
+ * it feeds the roofline figures of the benchmark.  *count receives the
+ * number of rows; at most `capacity` are written. */
+enum { NR_KERNEL_NAME_LEN = 48 };
+typedef struct nr_kernel_stat {
+    char name[NR_KERNEL_NAME_LEN];
+    int64_t launches;
+    double ms;
+    double bytes;
+    double flops;
+} nr_kernel_stat;
+NR_API int nr_batch_get_kernel_stats(const nr_batch* batch, nr_kernel_stat* out,
+                                     int32_t capacity, int32_t* count);
 
 /* ---- KernelDispatcher slots (tensors/kernel_dispatch.h:24-31) ----------- */
 /* Drop-in for batchDiodePhysics_generic (tensors/physics_tensors.cpp:19-51):

@@ -111,6 +111,14 @@ class NrInstanceStats(C.Structure):
     ]
 
 
+NR_KERNEL_NAME_LEN = 48
+
+
+class NrKernelStat(C.Structure):
+    _fields_ = [("name", C.c_char * NR_KERNEL_NAME_LEN), ("launches", C.c_int64), ("ms", C.c_double),
+                ("bytes", C.c_double), ("flops", C.c_double)]
+
+
 class NrTiming(C.Structure):
     _fields_ = [
         ("ms", C.c_double * NR_PHASE_COUNT), ("launches", C.c_int64 * NR_PHASE_COUNT),

@@ -36,7 +36,7 @@ EXPORTED = [
     "nr_topology_build", "nr_topology_destroy", "nr_topology_get_info", "nr_topology_export",
     "nr_batch_alloc", "nr_batch_free", "nr_batch_set_params", "nr_batch_set_seed",
     "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
-    "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing",
+    "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
     "nr_eval_diodes", "nr_eval_mosfets",
 ]
 
@@ -71,6 +71,8 @@ def lib():
         L.nr_batch_get_voltages.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_stats.argtypes = [_vp, C.POINTER(_abi.NrInstanceStats), C.c_int64]
         L.nr_batch_get_timing.argtypes = [_vp, C.POINTER(_abi.NrTiming)]
+        L.nr_batch_get_kernel_stats.argtypes = [_vp, C.POINTER(_abi.NrKernelStat), C.c_int32,
+                                                C.POINTER(C.c_int32)]
         L.nr_eval_diodes.argtypes = [_vp, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int32,
                                      _f64p, _f64p, _f64p]
         L.nr_eval_mosfets.argtypes = [_vp, C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
@@ -201,6 +203,16 @@ class Batch:
         _check(lib().nr_batch_get_timing(self.h, C.byref(t)), self.ctx)
         return {"ms": list(t.ms), "launches": list(t.launches), "factor_kernel_ms": t.factor_kernel_ms,
                 "factor_instances": t.factor_instances}
+
+    def kernel_stats(self):
+        """Per-kernel rows of the last solve run with record_timing=2:
+        {name: {launches, ms, bytes, flops}} (bytes and flops algorithmic)."""
+        n = C.c_int32()
+        _check(lib().nr_batch_get_kernel_stats(self.h, None, 0, C.byref(n)), self.ctx)
+        arr = (_abi.NrKernelStat * max(1, n.value))()
+        _check(lib().nr_batch_get_kernel_stats(self.h, arr, n.value, C.byref(n)), self.ctx)
+        return {r.name.decode(): {"launches": r.launches, "ms": r.ms, "bytes": r.bytes, "flops": r.flops}
+                for r in arr[: n.value]}
 
     def close(self):
         if getattr(self, "h", None):

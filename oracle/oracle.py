@@ -212,6 +212,16 @@ def lu_sparse(topo: Topology, a_vals, rhs):
     return rc, x
 
 
+def lu_multifrontal(topo: Topology, a_vals, rhs):
+    L = lib()
+    L.orc_lu_multifrontal.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, _f64p]
+    x = np.zeros(topo.n)
+    sp = C.c_double()
+    rc = L.orc_lu_multifrontal(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
+                               _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(sp))
+    return rc, x
+
+
 def seed(nl):
     c, keep = nl.to_c()
     v = np.zeros(nl.num_nodes)

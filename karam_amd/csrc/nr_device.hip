@@ -105,6 +105,10 @@ struct DevTopo {
     const int *children, *row_idx, *relmap, *fa_slot, *fa_li, *fa_lj;
     // small-front gather tables
     const int *tb_off, *tb;
+    // children of large fronts: number of update-block rows that land in the
+    // parent's pivot columns (a prefix, the maps are increasing), and the
+    // inverse map parent-local row -> child row (or -1), per child
+    const int *sn_pc, *inv_off, *inv;
     long long uS, uL, cbL, rcbL, nAl, zero_item;
 };
 
@@ -607,12 +611,26 @@ __device__ __forceinline__ void team_sync() {
     }
 }
 
-template <int TEAM>
-__global__ __launch_bounds__(256) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
-                                                      const int* __restrict__ sn_list, int n_sn, int rmax) {
+// Large fronts, one team per (front, instance).  The front is assembled in
+// LDS (in full when FULL, else only its panel: the c pivot rows (c x r), the
+// pivot columns below them (m x c) and the rhs), then eliminated in three
+// phases that reproduce, element by element, the operations and their order
+// of a right-looking elimination of the assembled front:
+//   (B) one wavefront factors the c x c diagonal block and the pivot rows'
+//       rhs, with wavefront-level barriers only;
+//   (C) every lane takes a column of the pivot rows right of the block (U12)
+//       or a row of the pivot columns below it (L21) and applies the c
+//       updates to it alone: no barrier;
+//   (D) the trailing m x m block (the front's update block) is formed in
+//       4x4 register tiles and streamed to HBM.  Without FULL its entries are
+//       gathered straight from the children's update blocks (every A entry of
+//       a front lies in a pivot row or column, so only children contribute).
+template <int TEAM, int NT, bool FULL>
+__global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+                                                     const int* __restrict__ sn_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
     extern __shared__ double smem[];
-    constexpr int TPB = 256 / TEAM;
+    constexpr int TPB = NT / TEAM;
     const int tib = threadIdx.x / TEAM;
     const int lane = threadIdx.x % TEAM;
     // tasks are front-major: the teams sharing a wavefront factor the same
@@ -626,13 +644,19 @@ __global__ __launch_bounds__(256) void k_factor_large(DevTopo T, DevBatch P, con
     if (slot >= nact) return;
     const int b = act[slot];
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    double* F = smem + (size_t)tib * (size_t)(rmax * rmax + rmax);
-    double* fr = F + r * r;
-    for (int e = lane; e < r * r + r; e += TEAM) F[e] = 0.0;
+    double* Pr = smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
+    // front-local (i, j) with i < c or j < c; rows c.. of the pivot columns
+    // have stride r in the full front, c in the panel
+    const int ls = FULL ? r : c;
+    double* Lc = FULL ? Pr + c * r : Pr + c * r;           // row i of L21 at Lc + i * ls
+    double* fr = FULL ? Pr + r * r : Pr + c * r + m * c;   // rhs, r
+    auto pidx = [&](int i, int j) { return i < c ? i * r + j : c * r + (i - c) * ls + j; };
+    const int np = FULL ? r * r + r : c * r + m * c + r;
+    for (int e = lane; e < np; e += TEAM) Pr[e] = 0.0;
     team_sync<TEAM>();
+    // (A) assembly: A entries, the pivot rows' rhs, then each child in order
     const double* Ab = P.Al + (size_t)b * T.nAl;
     {
-        // eight loads in flight per lane before the LDS updates
         const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
         for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
             double vals[8];
@@ -644,77 +668,141 @@ __global__ __launch_bounds__(256) void k_factor_large(DevTopo T, DevBatch P, con
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * TEAM;
-                if (e < a1) F[T.fa_li[e] * r + T.fa_lj[e]] += vals[u];
+                if (e < a1) Pr[pidx(T.fa_li[e], T.fa_lj[e])] += vals[u];
             }
         }
     }
     const double* rb = P.rhsl + (size_t)b * T.n;
     for (int k = lane; k < c; k += TEAM) fr[k] += rb[T.perm[f + k]];
     team_sync<TEAM>();
-    for (int ch = T.sn_child_begin[J]; ch < T.sn_child_end[J]; ++ch) {
+    const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
+    for (int ch = ch0; ch < ch1; ++ch) {
         const int C = T.children[ch];
-        const int mc = T.sn_nrows[C];
+        const int mc = T.sn_nrows[C], pc = FULL ? mc : T.sn_pc[C];
         const int* rel = T.relmap + T.sn_row_off[C];
         const double* cb = P.CBL + (size_t)b * T.cbL + T.sn_cb_off[C];
         const double* rcb = P.RCBL + (size_t)b * T.rcbL + T.sn_rcb_off[C];
-        const int mm = mc * mc;
-        for (int e0 = lane; e0 < mm; e0 += 8 * TEAM) {
+        // FULL: every entry; else rows a < pc (all columns) and columns q < pc
+        // of the remaining rows (the child's entries that land in the panel)
+        const int n1 = pc * mc, nn = n1 + (mc - pc) * pc;
+        for (int e0 = lane; e0 < nn; e0 += 8 * TEAM) {
             double vals[8];
+            int pos[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * TEAM;
-                vals[u] = (e < mm) ? cb[e] : 0.0;
+                int a = 0, q = 0;
+                if (e < n1) { a = e / mc; q = e - a * mc; }
+                else if (e < nn) { const int e2 = e - n1; a = pc + e2 / pc; q = e2 - (a - pc) * pc; }
+                vals[u] = (e < nn) ? cb[a * mc + q] : 0.0;
+                pos[u] = (e < nn) ? (FULL ? rel[a] * r + rel[q] : pidx(rel[a], rel[q])) : 0;
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int e = e0 + u * TEAM;
-                if (e < mm) {
-                    const int a = e / mc, q = e - a * mc;
-                    F[rel[a] * r + rel[q]] += vals[u];
-                }
-            }
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * TEAM < nn) Pr[pos[u]] += vals[u];
         }
         for (int a = lane; a < mc; a += TEAM) fr[rel[a]] += rcb[a];
         team_sync<TEAM>();
     }
-    // panel: the first c rows and columns, step by step
+    // (B) diagonal block and pivot-row rhs: at step k every lane owns a
+    // column j > k (SW consecutive columns per row stripe) and a stripe of the
+    // rows i > k, so the multipliers are broadcast reads and the row loop
+    // runs four independent updates at a time
+    constexpr int SW = TEAM < 64 ? TEAM : 64;
+    constexpr int NS = TEAM / SW;
+    const int jl = lane % SW, st = lane / SW;
     int sing = 0;
     for (int k = 0; k < c; ++k) {
-        const double piv = F[k * r + k];
+        const double piv = Pr[k * r + k];
         if (!(fabs(piv) >= 1e-25)) sing = 1;
-        for (int i = k + 1 + lane; i < r; i += TEAM) F[i * r + k] = F[i * r + k] / piv;
+        for (int i = k + 1 + lane; i < c; i += TEAM) Pr[i * r + k] = Pr[i * r + k] / piv;
         team_sync<TEAM>();
-        const int w = r - k - 1;          // columns k+1 .. r-1 of panel rows
-        const int rp = c - k - 1;         // panel rows k+1 .. c-1
-        const int cp = c - k - 1;         // panel columns k+1 .. c-1 of rows c .. r-1
-        const int n1 = rp * w, n2 = m * cp;
-        for (int e = lane; e < n1 + n2; e += TEAM) {
-            int i, j;
-            if (e < n1) { i = k + 1 + e / w; j = k + 1 + e % w; }
-            else { const int e2 = e - n1; i = c + e2 / cp; j = k + 1 + e2 % cp; }
-            F[i * r + j] = fma(-F[i * r + k], F[k * r + j], F[i * r + j]);
+        for (int j = k + 1 + jl; j < c; j += SW) {
+            const double u = Pr[k * r + j];
+            int i = k + 1 + st;
+            for (; i + 3 * NS < c; i += 4 * NS) {
+                const double l0 = Pr[i * r + k], l1 = Pr[(i + NS) * r + k];
+                const double l2 = Pr[(i + 2 * NS) * r + k], l3 = Pr[(i + 3 * NS) * r + k];
+                const double a0 = Pr[i * r + j], a1 = Pr[(i + NS) * r + j];
+                const double a2 = Pr[(i + 2 * NS) * r + j], a3 = Pr[(i + 3 * NS) * r + j];
+                Pr[i * r + j] = fma(-l0, u, a0);
+                Pr[(i + NS) * r + j] = fma(-l1, u, a1);
+                Pr[(i + 2 * NS) * r + j] = fma(-l2, u, a2);
+                Pr[(i + 3 * NS) * r + j] = fma(-l3, u, a3);
+            }
+            for (; i < c; i += NS) Pr[i * r + j] = fma(-Pr[i * r + k], u, Pr[i * r + j]);
         }
-        for (int i = k + 1 + lane; i < c; i += TEAM) fr[i] = fma(-F[i * r + k], fr[k], fr[i]);
+        for (int i = k + 1 + lane; i < c; i += TEAM) fr[i] = fma(-Pr[i * r + k], fr[k], fr[i]);
         team_sync<TEAM>();
     }
-    // trailing update, 4x4 register tiles, streamed to the update block
+    // (C) U12 columns and L21 rows, each by one lane
+    for (int t = lane; t < 2 * m; t += TEAM) {
+        if (t < m) {
+            const int j = c + t;
+            for (int k = 1; k < c; ++k) {
+                double x = Pr[k * r + j];
+                for (int p = 0; p < k; ++p) x = fma(-Pr[k * r + p], Pr[p * r + j], x);
+                Pr[k * r + j] = x;
+            }
+        } else {
+            double* Li = Lc + (t - m) * ls;
+            for (int k = 0; k < c; ++k) {
+                double x = Li[k];
+                for (int p = 0; p < k; ++p) x = fma(-Li[p], Pr[p * r + k], x);
+                Li[k] = x / Pr[k * r + k];
+            }
+        }
+    }
+    team_sync<TEAM>();
+    // (D) trailing block in 4x4 register tiles, streamed to the update block
     if (m > 0) {
-        double* cb = P.CBL + (size_t)b * T.cbL + T.sn_cb_off[J];
+        double* cbJ = P.CBL + (size_t)b * T.cbL + T.sn_cb_off[J];
+        const double* cbase = P.CBL + (size_t)b * T.cbL;
         const int mt = (m + 3) >> 2;
         for (int tt = lane; tt < mt * mt; tt += TEAM) {
             const int i0 = (tt / mt) * 4, j0 = (tt % mt) * 4;
             double acc[4][4];
+            if constexpr (FULL) {
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    acc[a][q] = (i0 + a < m && j0 + q < m) ? F[(c + i0 + a) * r + c + j0 + q] : 0.0;
+                    for (int q = 0; q < 4; ++q)
+                        acc[a][q] = (i0 + a < m && j0 + q < m) ? Pr[(c + i0 + a) * r + c + j0 + q] : 0.0;
+            } else {
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) acc[a][q] = 0.0;
+                for (int ch = ch0; ch < ch1; ++ch) {
+                    const int C = T.children[ch];
+                    const int mc = T.sn_nrows[C];
+                    const int* inv = T.inv + T.inv_off[C] + c;
+                    const double* cb = cbase + T.sn_cb_off[C];
+                    int ra[4], qa[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
+                        qa[a] = (j0 + a < m) ? inv[j0 + a] : -1;
+                    }
+                    double x[4][4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : 0.0;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            if (ra[a] >= 0 && qa[q] >= 0) acc[a][q] += x[a][q];
+                }
+            }
             for (int k = 0; k < c; ++k) {
                 double l[4], u[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) l[a] = (i0 + a < m) ? F[(c + i0 + a) * r + k] : 0.0;
+                for (int a = 0; a < 4; ++a) l[a] = (i0 + a < m) ? Lc[(i0 + a) * ls + k] : 0.0;
 #pragma unroll
-                for (int q = 0; q < 4; ++q) u[q] = (j0 + q < m) ? F[k * r + c + j0 + q] : 0.0;
+                for (int q = 0; q < 4; ++q) u[q] = (j0 + q < m) ? Pr[k * r + c + j0 + q] : 0.0;
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -724,29 +812,29 @@ __global__ __launch_bounds__(256) void k_factor_large(DevTopo T, DevBatch P, con
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
-                    if (i0 + a < m && j0 + q < m) cb[(i0 + a) * m + j0 + q] = acc[a][q];
+                    if (i0 + a < m && j0 + q < m) cbJ[(i0 + a) * m + j0 + q] = acc[a][q];
         }
         double* rcb = P.RCBL + (size_t)b * T.rcbL + T.sn_rcb_off[J];
         for (int i = lane; i < m; i += TEAM) {
             double acc = fr[c + i];
-            for (int k = 0; k < c; ++k) acc = fma(-F[(c + i) * r + k], fr[k], acc);
+            for (int k = 0; k < c; ++k) acc = fma(-Lc[i * ls + k], fr[k], acc);
             rcb[i] = acc;
         }
     }
     double* U = P.UL + (size_t)b * T.uL + T.sn_u_off[J];
-    for (int e = lane; e < c * r; e += TEAM) U[e] = F[e];
+    for (int e = lane; e < c * r; e += TEAM) U[e] = Pr[e];
     double* yb = P.yl + (size_t)b * T.n;
     for (int k = lane; k < c; k += TEAM) yb[f + k] = fr[k];
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
-template <int TEAM>
-__global__ __launch_bounds__(256) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
-                                                         int nact, const int* __restrict__ sn_list, int n_sn,
-                                                         int rmax) {
+template <int TEAM, int NT>
+__global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                        int nact, const int* __restrict__ sn_list, int n_sn,
+                                                        int rmax) {
 #pragma clang fp contract(off)
     extern __shared__ double smem[];
-    constexpr int TPB = 256 / TEAM;
+    constexpr int TPB = NT / TEAM;
     const int tib = threadIdx.x / TEAM;
     const int lane = threadIdx.x % TEAM;
     // tasks are front-major: the teams sharing a wavefront factor the same

@@ -225,6 +225,44 @@ typedef struct nr_result {
 NR_API int nr_batch_solve_dc(nr_batch* batch, const nr_solve_opts* opts,
                              nr_result* result);
 
+/* ---- transient (backward Euler, fixed step) ----------------------------- */
+/* A transient topology also holds the capacitor companion stamps in its
+ * pattern, as stepTransient stamps them (solvers/circuitsim_solver.cpp:
+ * 559-780, physics/circuitsim.h:2479-2505).  nr_topology_build builds the
+ * DC kind. */
+enum { NR_TOPO_DC = 0, NR_TOPO_TRANSIENT = 1 };
+NR_API int nr_topology_build_ex(nr_ctx* ctx, const nr_netlist* nl, int32_t kind,
+                                nr_topology** out);
+
+enum {
+    NR_TRAN_START_ZERO = 0,  /* zero voltages and capacitor history (the
+                                reference's start, D11)                     */
+    NR_TRAN_START_SEED = 1   /* the batch seed, e.g. a DC operating point   */
+};
+typedef struct nr_transient_opts {
+    double h;                /* fixed time step (s)                          */
+    int32_t steps;           /* steps to take                                */
+    int32_t max_iter;        /* NR iterations per step, MAX_NR_ITER = 50     */
+    int32_t start;           /* NR_TRAN_START_*                              */
+    int32_t n_probe;         /* nodes recorded at every step                 */
+    const int32_t* probe;    /* 1-based node ids, n_probe of them            */
+    int32_t record_timing;   /* as in nr_solve_opts                          */
+} nr_transient_opts;
+NR_API void nr_transient_opts_default(nr_transient_opts* opts);
+
+/* Fixed-step backward-Euler transient of every instance (the restated
+ * stepTransient: companion G = C/h with history (C/h) v_prev, PULSE sources
+ * at t_{n+1}, Newton from the previous step's solution with the 1e-12
+ * diagonal and no limiting or damping, converged by checkConvergence with
+ * the capacitor current in the KCL audit).  The batch's topology must be of
+ * kind NR_TOPO_TRANSIENT.  In nr_instance_stats, iterations count Newton
+ * iterations over all steps and gmin_steps_done counts completed steps. */
+NR_API int nr_batch_transient(nr_batch* batch, const nr_transient_opts* opts,
+                              nr_result* result);
+/* Probe voltages of the last transient: out[b][s][p], s = 0..steps (s = 0 is
+ * the start state); steps an instance did not complete hold NaN. */
+NR_API int nr_batch_get_waveform(const nr_batch* batch, double* out, int64_t count);
+
 /* Per-instance results.  out arrays are caller-owned host buffers. */
 NR_API int nr_batch_get_voltages(const nr_batch* batch, double* out, int64_t count);
 enum {

@@ -46,6 +46,11 @@ NR_PHASE_BACKSOLVE = 2
 NR_PHASE_UPDATE = 3
 NR_PHASE_COUNT = 4
 
+NR_TOPO_DC = 0
+NR_TOPO_TRANSIENT = 1
+NR_TRAN_START_ZERO = 0
+NR_TRAN_START_SEED = 1
+
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
@@ -93,6 +98,13 @@ class NrSolveOpts(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("precision", C.c_int32),
         ("seeded_start_step", C.c_int32), ("record_timing", C.c_int32), ("use_graph", C.c_int32),
+    ]
+
+
+class NrTransientOpts(C.Structure):
+    _fields_ = [
+        ("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
+        ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)), ("record_timing", C.c_int32),
     ]
 
 

@@ -110,6 +110,10 @@ struct DevTopo {
     // inverse map parent-local row -> child row (or -1), per child
     const int *sn_pc, *inv_off, *inv;
     long long uS, uL, cbL, rcbL, nAl, zero_item;
+    // transient: capacitors and PULSE sources (v1 v2 td tr tf pw per)
+    int nC;
+    const int *cap_n1, *cap_n2, *vsrc_is_pulse;
+    const double* vsrc_pulse;
 };
 
 struct InstState {
@@ -133,6 +137,12 @@ struct DevBatch {
     double *pool, *Us, *ys, *xs;
     double *Al, *rhsl, *UL, *CBL, *RCBL, *yl, *xl;
     double* seed;                // optional [B][n]
+    // transient: capacitances, capacitor voltages of the last accepted step,
+    // source values at the current time, the step h; vnow = the source values
+    // the KCL audit uses (vsrc for DC, vsrc_t in a transient)
+    double *cap, *cap_vprev, *vsrc_t;
+    const double* vnow;
+    double h;
     InstState* st;
     int* act;
     int* nact;
@@ -271,9 +281,14 @@ __device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const dou
             val = (volt(v, n, T.res_n1[idx]) - volt(v, n, T.res_n2[idx])) / P.R[(size_t)b * T.nR + idx];
             break;
         case karam::TK_VSRC:
-            val = (P.vsrc[(size_t)b * T.nV + idx] - (volt(v, n, T.vsrc_pos[idx]) - volt(v, n, T.vsrc_neg[idx]))) /
+            val = (P.vnow[(size_t)b * T.nV + idx] - (volt(v, n, T.vsrc_pos[idx]) - volt(v, n, T.vsrc_neg[idx]))) /
                   1e-3;
             break;
+        case karam::TK_CAP: {   // backward-Euler displacement current
+            const double vc = volt(v, n, T.cap_n1[idx]) - volt(v, n, T.cap_n2[idx]);
+            val = (P.cap[(size_t)b * T.nC + idx] / P.h) * (vc - P.cap_vprev[(size_t)b * T.nC + idx]);
+            break;
+        }
         case karam::TK_ISRC:
             val = P.isrc[(size_t)b * T.nI + idx];
             break;
@@ -302,13 +317,53 @@ __device__ double block_kcl(const DevTopo& T, const DevBatch& P, int b, const do
     return block_max(mx, red);
 }
 
+// linear rhs of node i in emission order: Norton V sources (at the values in
+// P.vnow), I sources, then the capacitor history currents (C/h) v_prev
+__device__ double rhs_lin_node(const DevTopo& T, const DevBatch& P, int b, int i) {
+#pragma clang fp contract(off)
+    double r = 0.0;
+    for (int k = T.rhs_lin_ptr[i]; k < T.rhs_lin_ptr[i + 1]; ++k) {
+        const int t = T.rhs_lin_term[k];
+        const int idx = term_idx(t);
+        double val;
+        switch (term_kind(t)) {
+        case karam::TK_VSRC: val = P.vnow[(size_t)b * T.nV + idx] * 1.0e3; break;
+        case karam::TK_CAP:
+            val = (P.cap[(size_t)b * T.nC + idx] / P.h) * P.cap_vprev[(size_t)b * T.nC + idx];
+            break;
+        default: val = P.isrc[(size_t)b * T.nI + idx]; break;
+        }
+        r = term_neg(t) ? r + (-val) : r + val;
+    }
+    return r;
+}
+
+// PULSE source value at time t (physics/circuitsim.h:2155-2170);
+// p = v1 v2 td tr tf pw per
+__device__ double pulse_value(const double* p, double time) {
+#pragma clang fp contract(off)
+    const double v1 = p[0], v2 = p[1], td = p[2], tr = p[3], tf = p[4], pw = p[5], per0 = p[6];
+    double t = time - td;
+    if (!(t > 0)) return v1;
+    const double per = per0 > 0 ? per0 : 1e99;
+    t = fmod(t, per);
+    if (t < tr) return v1 + (v2 - v1) * (t / tr);
+    if (t < tr + pw) return v2;
+    if (t < tr + pw + tf) return v2 + (v1 - v2) * ((t - tr - pw) / tf);
+    return v1;
+}
+
 // ---------------------------------------------------------------------------
 // kernels: set-up, evaluation, assembly
 // ---------------------------------------------------------------------------
 
-// seedInitialGuess (solvers/circuitsim_solver.cpp:32-116) or the caller's
-// seed; the per-instance linear part of A and rhs.  One block per instance.
-__global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_step) {
+// Start state of one instance (one block per instance) and the per-instance
+// linear part of A and rhs.  mode 0 (DC): seedInitialGuess
+// (solvers/circuitsim_solver.cpp:32-116) or the caller's seed; mode 1
+// (transient): zero voltages; mode 2 (transient): the caller's seed.  In a
+// transient the capacitor companions C/h join the linear part and the
+// capacitor history starts at the start state; the rhs is set per step.
+__global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_step, int mode) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
     const int n = T.n, Bp = P.Bp;
@@ -316,7 +371,7 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     __shared__ int use_seed;
     if (threadIdx.x == 0) {
         int ok = 0;
-        if (P.seed) {
+        if (P.seed && mode != 1) {
             const double* sv = P.seed + (size_t)b * n;
             for (int i = 0; i < n; ++i)
                 if (sv[i] == sv[i]) { ok = 1; break; }
@@ -326,6 +381,8 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     __syncthreads();
     if (use_seed) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = P.seed[(size_t)b * n + i];
+    } else if (mode != 0) {
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = 0.0;
     } else {
         for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = 0.0;
         __syncthreads();
@@ -356,22 +413,22 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         for (int k = T.lin_ptr[s]; k < T.lin_ptr[s + 1]; ++k) {
             const int t = T.lin_term[k];
             const int idx = term_idx(t);
-            double g = (term_kind(t) == karam::SK_RES) ? 1.0 / P.R[(size_t)b * T.nR + idx] : 1.0e3;
+            double g;
+            switch (term_kind(t)) {
+            case karam::SK_RES: g = 1.0 / P.R[(size_t)b * T.nR + idx]; break;
+            case karam::SK_CAP: g = P.cap[(size_t)b * T.nC + idx] / P.h; break;
+            default: g = 1.0e3; break;
+            }
             a = term_neg(t) ? a + (-g) : a + g;
         }
         P.glin[IX(s, b, Bp)] = a;
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double r = 0.0;
-        for (int k = T.rhs_lin_ptr[i]; k < T.rhs_lin_ptr[i + 1]; ++k) {
-            const int t = T.rhs_lin_term[k];
-            const int idx = term_idx(t);
-            double val = (term_kind(t) == karam::TK_VSRC) ? P.vsrc[(size_t)b * T.nV + idx] * 1.0e3
-                                                         : P.isrc[(size_t)b * T.nI + idx];
-            r = term_neg(t) ? r + (-val) : r + val;
-        }
-        P.rhslin[IX(i, b, Bp)] = r;
+    if (mode != 0) {
+        for (int i = threadIdx.x; i < T.nC; i += blockDim.x)
+            P.cap_vprev[(size_t)b * T.nC + i] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
     }
+    if (mode == 0)
+        for (int i = threadIdx.x; i < n; i += blockDim.x) P.rhslin[IX(i, b, Bp)] = rhs_lin_node(T, P, b, i);
     if (threadIdx.x == 0) {
         InstState s;
         s.step = use_seed ? seeded_step : 0;
@@ -381,6 +438,51 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         P.st[b] = s;
         P.act[b] = b;
     }
+}
+
+// Transient step set-up (one block per instance): source values at t_{n+1}
+// (PULSE sources from their waveform, the others at their DC value) and the
+// linear rhs with the capacitor history.
+__global__ __launch_bounds__(256) void k_step_begin(DevTopo T, DevBatch P, double t_next) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.x;
+    for (int i = threadIdx.x; i < T.nV; i += blockDim.x)
+        P.vsrc_t[(size_t)b * T.nV + i] =
+            T.vsrc_is_pulse[i] ? pulse_value(T.vsrc_pulse + 7 * i, t_next) : P.vsrc[(size_t)b * T.nV + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
+}
+
+// Transient step close (one block per instance): an instance whose step
+// converged shifts its capacitor history, records its probes and is
+// re-armed for the next step; after the last step it is finished.
+__global__ __launch_bounds__(256) void k_step_end(DevTopo T, DevBatch P, int step, int steps, int n_probe,
+                                                  const int* __restrict__ probe, double* __restrict__ wave) {
+    const int b = blockIdx.x;
+    const int n = T.n;
+    InstState st = P.st[b];
+    if (st.done != 2) return;
+    const double* v = P.v + (size_t)b * n;
+    for (int i = threadIdx.x; i < T.nC; i += blockDim.x)
+        P.cap_vprev[(size_t)b * T.nC + i] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
+    double* w = wave + ((size_t)b * (steps + 1) + step + 1) * n_probe;
+    for (int p = threadIdx.x; p < n_probe; p += blockDim.x) w[p] = volt(v, n, probe[p]);
+    if (threadIdx.x == 0) {
+        st.step = step + 1;
+        st.it = 0;
+        st.done = (step + 1 == steps) ? 1 : 0;
+        if (st.done) st.status = NR_STATUS_CONVERGED;
+        P.st[b] = st;
+    }
+}
+
+// probes of the start state
+__global__ __launch_bounds__(64) void k_wave_start(DevTopo T, DevBatch P, int steps, int n_probe,
+                                                   const int* __restrict__ probe, double* __restrict__ wave) {
+    const int b = blockIdx.x;
+    const double* v = P.v + (size_t)b * T.n;
+    double* w = wave + (size_t)b * (steps + 1) * n_probe;
+    for (int p = threadIdx.x; p < n_probe; p += blockDim.x) w[p] = volt(v, T.n, probe[p]);
 }
 
 // thread (item, lane): 4 items x 64 instances per block; blockIdx.y = group
@@ -421,7 +523,7 @@ __global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* 
 // Each value goes to the front that owns it: the interleaved small-front pool
 // or the instance-major large-front arrays.
 __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
-                                                  const double* __restrict__ gmin_tab) {
+                                                  const double* __restrict__ gmin_tab, int gmin_last) {
 #pragma clang fp contract(off)
     LANE_SETUP(4)
     if (item < T.nnz_a) {
@@ -437,7 +539,7 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
             }
             a = term_neg(t) ? a + (-g) : a + g;
         }
-        if (T.slot_diag[item]) a = a + gmin_tab[P.st[b].step];
+        if (T.slot_diag[item]) a = a + gmin_tab[min(P.st[b].step, gmin_last)];
         const int d = T.slot_dst[item];
         if (d & 1) P.Al[(size_t)b * T.nAl + (d >> 1)] = a;
         else P.pool[IX(d >> 1, b, Bp)] = a;
@@ -923,8 +1025,11 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
 // kernels: NR bookkeeping
 // ---------------------------------------------------------------------------
 // Per-instance NR control after a linear solve (one block per instance).
+// tran = 1: a transient step (no limiting, no damping, KCL_TOL = 1e-12, a
+// converged instance waits for the next step; solvers/circuitsim_solver.cpp:
+// 600-635 restated)
 __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
-                                                int nsteps) {
+                                                int nsteps, int tran) {
 #pragma clang fp contract(off)
     __shared__ double red[8];
     __shared__ int redi[8];
@@ -937,6 +1042,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
             st.total += st.it + 1;
             st.ref += 2 * (st.it + 1);
             st.prev_kcl = 1e9;
+            if (tran) st.kcl = 1e9;
             st.status = NR_STATUS_SINGULAR;
             st.done = 1;
             P.st[b] = st;
@@ -954,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     __syncthreads();
     // pnjlim in SoA order; dependency levels preserve the sequential result
     const double* Isb = P.Is + (size_t)b * T.nD;
-    for (int l = 0; l < T.n_pn_levels; ++l) {
+    for (int l = 0; !tran && l < T.n_pn_levels; ++l) {
         for (int k = T.pn_level_ptr[l] + threadIdx.x; k < T.pn_level_ptr[l + 1]; k += blockDim.x) {
             const int i = T.pn_diode[k];
             const double Vt = T.d_pn_vt[i];
@@ -974,7 +1080,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         __syncthreads();
     }
     double kcl = block_kcl(T, P, b, vn, red);
-    if (st.it > 0 && kcl >= st.prev_kcl) {
+    if (!tran && st.it > 0 && kcl >= st.prev_kcl) {
         for (int pass = 0; pass < 2; ++pass) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
             __syncthreads();
@@ -983,7 +1089,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         }
     }
     st.prev_kcl = kcl;
-    const double tol = (st.step == nsteps - 1) ? 1e-12 : 1.0;
+    const double tol = (tran || st.step == nsteps - 1) ? 1e-12 : 1.0;
     int conv;
     if (kcl < 1e-9) {
         conv = 1;
@@ -997,7 +1103,20 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         conv = ok && !(kcl > tol);
     }
     for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = vn[i];
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && tran) {
+        st.kcl = kcl;                         // the last iterate's audit
+        if (conv) {
+            st.total += st.it + 1;
+            st.ref += st.it + 1;
+            st.done = 2;                      // step converged
+        } else if (++st.it >= max_iter) {
+            st.total += max_iter;
+            st.ref += max_iter;
+            st.status = NR_STATUS_NOT_CONVERGED;
+            st.done = 1;
+        }
+        P.st[b] = st;
+    } else if (threadIdx.x == 0) {
         if (conv) {
             st.total += st.it + 1;
             st.ref += st.it + 1;

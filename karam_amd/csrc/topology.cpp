@@ -105,7 +105,7 @@ void tensorize(const nr_netlist* nl, SoA& s) {
 namespace {
 
 // DC stamp emission order of stampSoABlock (h = 0) + gmin diagonal.
-void emit_stamps(const nr_netlist* nl, const SoA& s, std::vector<Stamp>& out) {
+void emit_stamps(const nr_netlist* nl, const SoA& s, std::vector<Stamp>& out, bool transient) {
     out.clear();
     auto st = [&](int32_t r, int32_t c, int32_t k, int32_t idx, int32_t sg) {
         out.push_back(Stamp{r, c, k, idx, sg});
@@ -125,6 +125,15 @@ void emit_stamps(const nr_netlist* nl, const SoA& s, std::vector<Stamp>& out) {
             st(p - 1, q - 1, SK_VSRC, i, -1); st(q - 1, p - 1, SK_VSRC, i, -1);
         } else if (p != 0) st(p - 1, p - 1, SK_VSRC, i, 1);
         else if (q != 0) st(q - 1, q - 1, SK_VSRC, i, 1);
+    }
+    // capacitor companions follow the sources (physics/circuitsim.h:2479-2505)
+    for (int32_t i = 0; transient && i < nl->n_cap; ++i) {
+        int32_t a = nl->cap_n1[i], b = nl->cap_n2[i];
+        if (a != 0 && b != 0) {
+            st(a - 1, a - 1, SK_CAP, i, 1); st(b - 1, b - 1, SK_CAP, i, 1);
+            st(a - 1, b - 1, SK_CAP, i, -1); st(b - 1, a - 1, SK_CAP, i, -1);
+        } else if (a != 0) st(a - 1, a - 1, SK_CAP, i, 1);
+        else if (b != 0) st(b - 1, b - 1, SK_CAP, i, 1);
     }
     const int32_t nD = (int32_t)s.d_a.size();
     for (int32_t i = 0; i < nD; ++i) {
@@ -198,7 +207,7 @@ struct MinFill {
 
 }  // namespace
 
-int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
+int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool transient) {
     auto t0 = std::chrono::steady_clock::now();
     if (!nl || nl->num_nodes <= 0) { err = "netlist has no nodes"; return NR_ERR_INVALID; }
     const int32_t n = nl->num_nodes;
@@ -233,12 +242,22 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
     t.isrc_neg.assign(nl->isrc_neg, nl->isrc_neg + nl->n_isrc);
     t.isrc_amps.assign(nl->isrc_amps, nl->isrc_amps + nl->n_isrc);
     t.dio_anode_nl.assign(nl->dio_anode, nl->dio_anode + nl->n_diode);
+    t.cap_n1.assign(nl->cap_n1, nl->cap_n1 + nl->n_cap);
+    t.cap_n2.assign(nl->cap_n2, nl->cap_n2 + nl->n_cap);
+    t.cap_farads.assign(nl->cap_farads, nl->cap_farads + nl->n_cap);
+    t.vsrc_is_pulse.assign(nl->n_vsrc, 0);
+    t.vsrc_pulse.assign((size_t)nl->n_vsrc * 7, 0.0);
+    for (int32_t i = 0; i < nl->n_vsrc && nl->vsrc_is_pulse && nl->vsrc_pulse; ++i) {
+        t.vsrc_is_pulse[i] = nl->vsrc_is_pulse[i] ? 1 : 0;
+        for (int q = 0; q < 7; ++q) t.vsrc_pulse[(size_t)i * 7 + q] = nl->vsrc_pulse[(size_t)i * 7 + q];
+    }
+    t.transient = transient;
     tensorize(nl, t.soa);
     const SoA& s = t.soa;
     const int32_t nD = (int32_t)s.d_a.size(), nM = (int32_t)s.m_d.size();
 
     // ---- stamps, CSR pattern, stamp map -------------------------------------
-    emit_stamps(nl, s, t.stamps);
+    emit_stamps(nl, s, t.stamps, transient);
     const int32_t S = (int32_t)t.stamps.size();
     std::vector<int32_t> order(S);
     std::iota(order.begin(), order.end(), 0);
@@ -271,7 +290,7 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
     for (int32_t k = 0; k < S; ++k) {
         const Stamp& x = t.stamps[k];
         int32_t sl = t.stamp_slot[k];
-        if (x.kind == SK_RES || x.kind == SK_VSRC) t.lin_ptr[sl + 1]++;
+        if (x.kind == SK_RES || x.kind == SK_VSRC || x.kind == SK_CAP) t.lin_ptr[sl + 1]++;
         else if (x.kind == SK_DIODE || x.kind == SK_MOS_GDS || x.kind == SK_MOS_GM) t.nl_ptr[sl + 1]++;
         else if (x.kind == SK_GMIN) t.slot_is_diag[sl] = 1;
     }
@@ -283,7 +302,8 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
         for (int32_t k = 0; k < S; ++k) {      // emission order
             const Stamp& x = t.stamps[k];
             int32_t sl = t.stamp_slot[k];
-            if (x.kind == SK_RES || x.kind == SK_VSRC) t.lin_term[fl[sl]++] = pack_term(x.kind, x.sign, x.idx);
+            if (x.kind == SK_RES || x.kind == SK_VSRC || x.kind == SK_CAP)
+                t.lin_term[fl[sl]++] = pack_term(x.kind, x.sign, x.idx);
             else if (x.kind == SK_DIODE || x.kind == SK_MOS_GDS || x.kind == SK_MOS_GM)
                 t.nl_term[fn[sl]++] = pack_term(x.kind, x.sign, x.idx);
         }
@@ -301,6 +321,11 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
             int32_t p = nl->isrc_pos[i], q = nl->isrc_neg[i];
             if (p != 0) lin[p - 1].push_back(pack_term(TK_ISRC, -1, i));
             if (q != 0) lin[q - 1].push_back(pack_term(TK_ISRC, 1, i));
+        }
+        for (int32_t i = 0; transient && i < nl->n_cap; ++i) {   // history current
+            int32_t a = nl->cap_n1[i], b = nl->cap_n2[i];
+            if (a != 0) lin[a - 1].push_back(pack_term(TK_CAP, 1, i));
+            if (b != 0) lin[b - 1].push_back(pack_term(TK_CAP, -1, i));
         }
         for (int32_t i = 0; i < nD; ++i) {
             if (s.d_a[i] != 0) nlt[s.d_a[i] - 1].push_back(pack_term(TK_DIODE, -1, i));
@@ -342,6 +367,9 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err) {
         }
         for (int32_t i = 0; i < nl->n_mos; ++i) {
             add(nl->mos_d[i], pack_term(TK_MOS, -1, mpos[i])); add(nl->mos_s[i], pack_term(TK_MOS, 1, mpos[i]));
+        }
+        for (int32_t i = 0; transient && i < nl->n_cap; ++i) {   // displacement current (:1460-1490)
+            add(nl->cap_n1[i], pack_term(TK_CAP, -1, i)); add(nl->cap_n2[i], pack_term(TK_CAP, 1, i));
         }
         t.kcl_ptr.assign(n + 1, 0);
         for (int32_t i = 0; i < n; ++i) {

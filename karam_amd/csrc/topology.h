@@ -26,7 +26,8 @@ enum StampKind : int32_t {
     SK_MOS_GDS = 3,  // +/- gds of SoA MOSFET idx
     SK_MOS_GM = 4,   // +/- gm of SoA MOSFET idx
     SK_MOS_GMB = 5,  // +/- gmb (identically 0 for the generic kernel)
-    SK_GMIN = 6      // + gmin
+    SK_GMIN = 6,     // + gmin
+    SK_CAP = 7       // +/- C/h of capacitor idx (transient topologies only)
 };
 
 // rhs / KCL contribution sources
@@ -35,7 +36,8 @@ enum TermKind : int32_t {
     TK_ISRC = 1,     // rhs: -/+ I     ; KCL: -/+ I
     TK_DIODE = 2,    // rhs: -/+ ieq (SoA idx) ; KCL: -/+ i_d (netlist idx)
     TK_MOS = 3,      // rhs: -/+ ieq (SoA idx) ; KCL: -/+ ids (SoA idx)
-    TK_RES = 4       // KCL only: -/+ (v1 - v2)/R
+    TK_RES = 4,      // KCL only: -/+ (v1 - v2)/R
+    TK_CAP = 5       // rhs: +/- (C/h) v_prev ; KCL: -/+ (C/h)(v - v_prev)  (transient)
 };
 
 struct Stamp {
@@ -80,17 +82,21 @@ struct Topology {
     int32_t n_res = 0, n_vsrc = 0, n_isrc = 0, n_cap = 0, n_dio_nl = 0, n_mos_nl = 0;
     std::vector<int32_t> res_n1, res_n2, vsrc_pos, vsrc_neg, isrc_pos, isrc_neg, dio_anode_nl;
     std::vector<double> res_ohms, vsrc_volts, isrc_amps;
+    std::vector<int32_t> cap_n1, cap_n2, vsrc_is_pulse;
+    std::vector<double> cap_farads, vsrc_pulse;   // pulse: 7 values per source
+    // transient topology: capacitor companion stamps are part of the pattern
+    bool transient = false;
 
     // stamps in emission order, CSR pattern of A and the stamp map
     std::vector<Stamp> stamps;
     std::vector<int32_t> row_ptr, col_idx, stamp_slot;
     int32_t nnz_a = 0;
     // per-slot stamp lists (emission order): linear part then nonlinear part
-    std::vector<int32_t> lin_ptr, lin_term;     // SK_RES / SK_VSRC
+    std::vector<int32_t> lin_ptr, lin_term;     // SK_RES / SK_VSRC / SK_CAP
     std::vector<int32_t> nl_ptr, nl_term;       // SK_DIODE / SK_MOS_* (gmb skipped)
     std::vector<int32_t> slot_is_diag;          // gmin added last
     // per-node rhs lists (emission order)
-    std::vector<int32_t> rhs_lin_ptr, rhs_lin_term;   // TK_VSRC / TK_ISRC
+    std::vector<int32_t> rhs_lin_ptr, rhs_lin_term;   // TK_VSRC / TK_ISRC / TK_CAP
     std::vector<int32_t> rhs_nl_ptr, rhs_nl_term;     // TK_DIODE / TK_MOS (SoA)
     // per-node KCL lists (netlist order of calculatePhysicalResiduals)
     std::vector<int32_t> kcl_ptr, kcl_term;
@@ -120,8 +126,9 @@ struct Topology {
     double analyse_ms = 0.0;
 };
 
-// Build; returns NR_OK or an error code with a message in `err`.
-int build_topology(const nr_netlist* nl, Topology& t, std::string& err);
+// Build; returns NR_OK or an error code with a message in `err`.  A
+// transient topology also carries the capacitor companion stamps.
+int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool transient = false);
 
 // tensorizeNetlist order and parameters (tensors/physics_tensors.cpp:305-493).
 void tensorize(const nr_netlist* nl, SoA& s);

@@ -33,11 +33,13 @@ _vp = C.c_void_p
 # every symbol the C header declares (checked by the CPU test suite)
 EXPORTED = [
     "nr_ctx_create", "nr_ctx_destroy", "nr_last_error", "nr_version",
-    "nr_topology_build", "nr_topology_destroy", "nr_topology_get_info", "nr_topology_export",
+    "nr_topology_build", "nr_topology_build_ex", "nr_topology_destroy", "nr_topology_get_info",
+    "nr_topology_export",
     "nr_batch_alloc", "nr_batch_free", "nr_batch_set_params", "nr_batch_set_seed",
     "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
     "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
     "nr_eval_diodes", "nr_eval_mosfets",
+    "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform",
 ]
 
 
@@ -57,6 +59,11 @@ def lib():
         L.nr_ctx_create.argtypes = [C.POINTER(_abi.NrDeviceCfg), C.POINTER(_vp)]
         L.nr_ctx_destroy.argtypes = [_vp]
         L.nr_topology_build.argtypes = [_vp, C.POINTER(_abi.NrNetlist), C.POINTER(_vp)]
+        L.nr_topology_build_ex.argtypes = [_vp, C.POINTER(_abi.NrNetlist), C.c_int32, C.POINTER(_vp)]
+        L.nr_transient_opts_default.argtypes = [C.POINTER(_abi.NrTransientOpts)]
+        L.nr_transient_opts_default.restype = None
+        L.nr_batch_transient.argtypes = [_vp, C.POINTER(_abi.NrTransientOpts), C.POINTER(_abi.NrResult)]
+        L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_topology_destroy.argtypes = [_vp]
         L.nr_topology_get_info.argtypes = [_vp, C.POINTER(_abi.NrTopologyInfo)]
         L.nr_topology_export.argtypes = [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]
@@ -115,11 +122,13 @@ class Context:
 class Topology:
     """Symbolic analysis of a netlist (host side; needs no GPU)."""
 
-    def __init__(self, nl, ctx: Context | None = None):
+    def __init__(self, nl, ctx: Context | None = None, transient: bool = False):
         self.nl = nl
+        self.transient = bool(transient)
         self._c, self._keep = nl.to_c()
         h = _vp()
-        rc = lib().nr_topology_build(ctx.h if ctx else None, C.byref(self._c), C.byref(h))
+        kind = _abi.NR_TOPO_TRANSIENT if transient else _abi.NR_TOPO_DC
+        rc = lib().nr_topology_build_ex(ctx.h if ctx else None, C.byref(self._c), kind, C.byref(h))
         _check(rc, ctx)
         self.h = h
         info = _abi.NrTopologyInfo()
@@ -180,6 +189,28 @@ class Batch:
         r = _abi.NrResult()
         _check(lib().nr_batch_solve_dc(self.h, C.byref(o), C.byref(r)), self.ctx)
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def transient(self, h=1e-9, steps=200, max_iter=50, start=_abi.NR_TRAN_START_ZERO, probes=None,
+                  record_timing=0):
+        """Fixed-step backward-Euler transient of every instance (needs a
+        transient topology).  probes: 1-based node ids recorded each step."""
+        o = _abi.NrTransientOpts()
+        lib().nr_transient_opts_default(C.byref(o))
+        self._probes = np.ascontiguousarray(probes if probes is not None else [], dtype=np.int32)
+        o.h, o.steps, o.max_iter, o.start = float(h), int(steps), int(max_iter), int(start)
+        o.n_probe = len(self._probes)
+        o.probe = _p32(self._probes) if len(self._probes) else None
+        o.record_timing = record_timing
+        r = _abi.NrResult()
+        _check(lib().nr_batch_transient(self.h, C.byref(o), C.byref(r)), self.ctx)
+        self._wave_shape = (self.B, int(steps) + 1, len(self._probes))
+        return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def waveform(self):
+        """Probe voltages of the last transient, [B, steps + 1, n_probe]."""
+        out = np.zeros(self._wave_shape)
+        _check(lib().nr_batch_get_waveform(self.h, _p64(out), out.size), self.ctx)
+        return out
 
     def voltages(self):
         n = self.topo.info["n"]

@@ -250,6 +250,7 @@ struct orc_topo {
     int32_t *sn_first, *sn_ncols, *sn_nrows, *sn_row_off, *sn_rows;
     int32_t *sn_child_ptr, *sn_child;
     int32_t *sn_a_ptr, *sn_a_slot, *sn_a_li, *sn_a_lj;
+    int32_t transient;             /* capacitor companions in the pattern */
 };
 
 /* growable int vector */
@@ -277,8 +278,18 @@ static inline void emit(emitter* e, int32_t r, int32_t c, double val) {
     else { iv_push(&e->rows, r); iv_push(&e->cols, c); }
 }
 
+/* Transient step context (restated stepTransient, solvers/circuitsim_solver.cpp:
+ * 559-780): step h, source values at t_{n+1}, capacitor voltages of the last
+ * accepted step.  NULL for the DC solve. */
+typedef struct {
+    double h;
+    const double* vsrc;     /* [n_vsrc] */
+    const double* vprev;    /* [n_cap]  */
+    int coords_only;        /* pattern build: emit the capacitor coordinates */
+} tran_ctx;
+
 static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
-                      double gmin, emitter* e) {
+                      double gmin, emitter* e, const tran_ctx* tc) {
     const int32_t n = nl->num_nodes;
     double* rhs = e->rhs;
     /* (1) resistors :2416-2438 */
@@ -294,7 +305,7 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     /* (2) voltage sources, Norton 1e3 S :2440-2467 */
     for (int32_t i = 0; i < nl->n_vsrc; ++i) {
         int32_t p = nl->vsrc_pos[i], q = nl->vsrc_neg[i];
-        double V = nl->vsrc_volts[i];
+        double V = (tc && tc->vsrc) ? tc->vsrc[i] : nl->vsrc_volts[i];
         double g = 1.0e3, ieq = V * g;
         if (p != 0 && q != 0) {
             emit(e, p - 1, p - 1, g); emit(e, q - 1, q - 1, g);
@@ -313,6 +324,23 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
             double I = nl->isrc_amps[i];
             if (p != 0) rhs[p - 1] -= I;
             if (q != 0) rhs[q - 1] += I;
+        }
+    }
+    /* (4) capacitors, backward-Euler companion G = C/h with history current
+     * G v_prev (restating :2479-2505 with the Gear-1 companion of
+     * math/integration_method.h:83-115) */
+    for (int32_t i = 0; tc && i < nl->n_cap; ++i) {
+        int32_t a = nl->cap_n1[i], b = nl->cap_n2[i];
+        double G = 0.0, ih = 0.0;
+        if (!tc->coords_only) { G = nl->cap_farads[i] / tc->h; ih = G * tc->vprev[i]; }
+        if (a != 0 && b != 0) {
+            emit(e, a - 1, a - 1, G); emit(e, b - 1, b - 1, G);
+            emit(e, a - 1, b - 1, -G); emit(e, b - 1, a - 1, -G);
+        } else if (a != 0) emit(e, a - 1, a - 1, G);
+        else if (b != 0) emit(e, b - 1, b - 1, G);
+        if (rhs) {
+            if (a != 0) rhs[a - 1] += ih;
+            if (b != 0) rhs[b - 1] -= ih;
         }
     }
     /* (6) device evaluation :2535-2537, (7) diodes :2543-2576 */
@@ -673,14 +701,18 @@ static int cmp_key_idx(const void* a, const void* b) {
     return (x > y) - (x < y);
 }
 
-int orc_topology_build(const nr_netlist* nl, orc_topo** out) {
+int orc_topology_build(const nr_netlist* nl, orc_topo** out) { return orc_topology_build_ex(nl, 0, out); }
+
+int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** out) {
     if (!nl || nl->num_nodes <= 0) return NR_ERR_INVALID;
     orc_topo* t = (orc_topo*)xcalloc(1, sizeof(orc_topo));
     t->n = nl->num_nodes;
+    t->transient = transient ? 1 : 0;
     t->soa = orc_tensorize(nl);
     emitter e;
     memset(&e, 0, sizeof(e));
-    stamp_all(nl, t->soa, NULL, 0.0, &e);
+    tran_ctx coords = {0.0, NULL, NULL, 1};
+    stamp_all(nl, t->soa, NULL, 0.0, &e, transient ? &coords : NULL);
     t->n_stamps = e.rows.n;
     t->st_row = e.rows.a; t->st_col = e.cols.a;
     const int32_t S = t->n_stamps, n = t->n;
@@ -759,20 +791,20 @@ int64_t orc_topology_export(const orc_topo* t, int32_t what, int32_t* out, int64
 /* stamping + assembly of one iteration                                     */
 /* ------------------------------------------------------------------------ */
 static int stamp_with_soa(const nr_netlist* nl, const orc_soa* s, const orc_topo* t,
-                          const double* v, double gmin, double* vals, double* rhs) {
+                          const double* v, double gmin, double* vals, double* rhs, const tran_ctx* tc) {
     emitter e;
     memset(&e, 0, sizeof(e));
     e.vals = vals;
     e.rhs = rhs;
     memset(rhs, 0, sizeof(double) * (size_t)nl->num_nodes);
-    stamp_all(nl, s, v, gmin, &e);
+    stamp_all(nl, s, v, gmin, &e, tc);
     return e.nv == t->n_stamps ? NR_OK : NR_ERR_TOPOLOGY;
 }
 
 int orc_stamp(const nr_netlist* nl, const orc_topo* t, const double* v,
               double gmin, double* vals, double* rhs) {
     orc_soa* s = orc_tensorize(nl);
-    int rc = stamp_with_soa(nl, s, t, v, gmin, vals, rhs);
+    int rc = stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
     orc_soa_free(s);
     return rc;
 }
@@ -1083,7 +1115,7 @@ static double diode_i_clamped(double vd, double Is, double N, double Vt) {
 }
 
 static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
-                      double* res) {
+                      double* res, const tran_ctx* tc) {
     const int32_t n = nl->num_nodes;
     for (int32_t i = 0; i < n; ++i) res[i] = 0.0;
 #define ADDRES(node, val) do { int32_t nd_ = (node); if (nd_ > 0 && nd_ <= n) res[nd_ - 1] += (val); } while (0)
@@ -1092,7 +1124,8 @@ static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
         ADDRES(nl->res_n1[i], -ir); ADDRES(nl->res_n2[i], ir);
     }
     for (int32_t i = 0; i < nl->n_vsrc; ++i) {
-        double isrc = (nl->vsrc_volts[i] - (volt(v, n, nl->vsrc_pos[i]) - volt(v, n, nl->vsrc_neg[i]))) / 1e-3;
+        double V = (tc && tc->vsrc) ? tc->vsrc[i] : nl->vsrc_volts[i];
+        double isrc = (V - (volt(v, n, nl->vsrc_pos[i]) - volt(v, n, nl->vsrc_neg[i]))) / 1e-3;
         ADDRES(nl->vsrc_pos[i], isrc); ADDRES(nl->vsrc_neg[i], -isrc);
     }
     for (int32_t i = 0; i < nl->n_isrc; ++i) {
@@ -1115,6 +1148,12 @@ static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
         }
         free(pos);
     }
+    /* capacitor displacement current, Gear-1 branch (physics/circuitsim.h:1460-1490) */
+    for (int32_t i = 0; tc && i < nl->n_cap; ++i) {
+        double vc = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
+        double ic = (nl->cap_farads[i] / tc->h) * (vc - tc->vprev[i]);
+        ADDRES(nl->cap_n1[i], -ic); ADDRES(nl->cap_n2[i], ic);
+    }
 #undef ADDRES
     double mx = 0.0;
     for (int32_t i = 0; i < n; ++i) mx = dmax_ref(mx, fabs(res[i]));
@@ -1125,7 +1164,7 @@ double orc_kcl(const nr_netlist* nl, const orc_topo* t, const double* v, double*
     (void)t;
     orc_soa* s = orc_tensorize(nl);
     double* res = per_node ? per_node : (double*)xmalloc(sizeof(double) * (size_t)nl->num_nodes);
-    double m = kcl_soa(nl, s, v, res);
+    double m = kcl_soa(nl, s, v, res, NULL);
     if (!per_node) free(res);
     orc_soa_free(s);
     return m;
@@ -1179,7 +1218,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         int gconv = 0, sing_step = 0;
         int32_t it = 0;
         for (; it < o.max_iter; ++it) {
-            stamp_with_soa(nl, s, t, v, gmin, vals, rhs);
+            stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
             orc_assemble(t, vals, av);
             int rc = (o.lu_kind == ORC_LU_DENSE)
                          ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
@@ -1193,13 +1232,13 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
             }
             for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
             apply_pn_limits_soa(nl, s, vn, v);
-            double kcl = kcl_soa(nl, s, vn, res);
+            double kcl = kcl_soa(nl, s, vn, res, NULL);
             if (it > 0 && kcl >= final_kcl) {
                 for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
-                kcl = kcl_soa(nl, s, vn, res);
+                kcl = kcl_soa(nl, s, vn, res, NULL);
                 if (kcl >= final_kcl) {
                     for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
-                    kcl = kcl_soa(nl, s, vn, res);
+                    kcl = kcl_soa(nl, s, vn, res, NULL);
                 }
             }
             final_kcl = kcl;
@@ -1219,7 +1258,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         if (k == o.gmin_steps - 1 && gconv) { converged = 1; break; }
         if (!gconv && !(final_kcl < 1.0)) break;
     }
-    double fk = kcl_soa(nl, s, v, res);
+    double fk = kcl_soa(nl, s, v, res, NULL);
     out->iterations = total;
     out->ref_iterations = ref_total;
     out->status = converged ? NR_STATUS_CONVERGED : (singular ? NR_STATUS_SINGULAR : NR_STATUS_NOT_CONVERGED);
@@ -1253,6 +1292,120 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
         }
         orc_solve_dc(&inst, t, opts, v_out + (size_t)b * n, &outs[b]);
+    }
+    return NR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a16: fixed-step backward-Euler transient, stepTransient restated          */
+/* (solvers/circuitsim_solver.cpp:559-780; D10-D13)                          */
+/* ------------------------------------------------------------------------ */
+/* PULSE value at time `time` (physics/circuitsim.h:2155-2170);
+ * p = v1 v2 td tr tf pw per */
+double orc_pulse_value(const double* p, double time) {
+    double v1 = p[0], v2 = p[1], td = p[2], tr = p[3], tf = p[4], pw = p[5];
+    double t = time - td;
+    if (!(t > 0)) return v1;
+    double per = p[6] > 0 ? p[6] : 1e99;
+    t = fmod(t, per);
+    if (t < tr) return v1 + (v2 - v1) * (t / tr);
+    if (t < tr + pw) return v2;
+    if (t < tr + pw + tf) return v2 + (v1 - v2) * ((t - tr - pw) / tf);
+    return v1;
+}
+
+void orc_tran_opts_default(orc_tran_opts* o) {
+    o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_MULTIFRONTAL;
+}
+
+int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_opts* opts, double* v_io,
+                        int32_t n_probe, const int32_t* probe, double* wave, orc_tran_out* out) {
+    orc_tran_opts o;
+    if (opts) o = *opts; else orc_tran_opts_default(&o);
+    memset(out, 0, sizeof(*out));
+    const int32_t n = nl->num_nodes;
+    if (n <= 0 || !t || !t->transient || !(o.h > 0)) return NR_ERR_INVALID;
+    orc_soa* s = orc_tensorize(nl);
+    double* v = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* vn = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* rhs = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* res = (double*)xmalloc(sizeof(double) * (size_t)n);
+    double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    double* vs = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_vsrc + 1));
+    double* vprev = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    /* start state (D11): zero voltages and history, or the caller's state */
+    if (o.start) memcpy(v, v_io, sizeof(double) * (size_t)n);
+    else memset(v, 0, sizeof(double) * (size_t)n);
+    for (int32_t i = 0; i < nl->n_cap; ++i) vprev[i] = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
+    if (wave) {
+        for (int64_t k = 0; k < (int64_t)(o.steps + 1) * n_probe; ++k) wave[k] = NAN;
+        for (int32_t p = 0; p < n_probe; ++p) wave[p] = volt(v, n, probe[p]);
+    }
+    tran_ctx tc = {o.h, vs, vprev, 0};
+    int32_t total = 0, steps_done = 0, status = NR_STATUS_CONVERGED;
+    double kcl = 0.0;
+    for (int32_t step = 0; step < o.steps; ++step) {
+        const double tnext = (double)(step + 1) * o.h;
+        for (int32_t i = 0; i < nl->n_vsrc; ++i)
+            vs[i] = (nl->vsrc_is_pulse && nl->vsrc_is_pulse[i] && nl->vsrc_pulse)
+                        ? orc_pulse_value(nl->vsrc_pulse + 7 * (size_t)i, tnext) : nl->vsrc_volts[i];
+        int conv = 0;
+        int32_t it = 0;
+        for (; it < o.max_iter; ++it) {
+            stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
+            orc_assemble(t, vals, av);
+            int rc = (o.lu_kind == ORC_LU_DENSE)
+                         ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
+                         : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
+                                                        : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+            if (rc != NR_OK) { status = NR_STATUS_SINGULAR; total += it + 1; kcl = 1e9; break; }
+            for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
+            kcl = kcl_soa(nl, s, vn, res, &tc);
+            int c = orc_check_convergence(n, vn, v, kcl, 1e-12);
+            memcpy(v, vn, sizeof(double) * (size_t)n);
+            if (c) { conv = 1; total += it + 1; break; }
+        }
+        if (status == NR_STATUS_SINGULAR) break;
+        if (!conv) { total += o.max_iter; status = NR_STATUS_NOT_CONVERGED; break; }
+        for (int32_t i = 0; i < nl->n_cap; ++i) vprev[i] = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
+        if (wave)
+            for (int32_t p = 0; p < n_probe; ++p) wave[(size_t)(step + 1) * n_probe + p] = volt(v, n, probe[p]);
+        ++steps_done;
+    }
+    out->iterations = total;
+    out->status = status;
+    out->steps_done = steps_done;
+    out->kcl = kcl;
+    memcpy(v_io, v, sizeof(double) * (size_t)n);
+    free(v); free(vn); free(rhs); free(res); free(vals); free(av); free(vs); free(vprev);
+    orc_soa_free(s);
+    return NR_OK;
+}
+
+int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                              const orc_tran_opts* opts, int32_t threads, double* v_io, int32_t n_probe,
+                              const int32_t* probe, double* wave, orc_tran_out* outs) {
+    if (!t || B <= 0) return NR_ERR_INVALID;
+    const int32_t n = nl->num_nodes;
+    const int32_t steps = opts ? opts->steps : 200;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+    for (int32_t b = 0; b < B; ++b) {
+        nr_netlist inst = *nl;
+        if (fields) {
+            if (fields[NR_FIELD_RES_OHMS]) inst.res_ohms = fields[NR_FIELD_RES_OHMS] + (size_t)b * nl->n_res;
+            if (fields[NR_FIELD_DIO_IS]) inst.dio_is = fields[NR_FIELD_DIO_IS] + (size_t)b * nl->n_diode;
+            if (fields[NR_FIELD_MOS_KP]) inst.mos_kp = fields[NR_FIELD_MOS_KP] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_MOS_VTO]) inst.mos_vto = fields[NR_FIELD_MOS_VTO] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
+            if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
+            if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+        }
+        orc_solve_transient(&inst, t, opts, v_io + (size_t)b * n, n_probe, probe,
+                            wave ? wave + (size_t)b * (steps + 1) * n_probe : NULL, &outs[b]);
     }
     return NR_OK;
 }

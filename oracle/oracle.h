@@ -109,6 +109,32 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
                        const double* const* fields, const orc_dc_opts* opts,
                        int32_t threads, double* v_out, orc_dc_out* outs);
 
+/* --- a16: fixed-step backward-Euler transient ----------------------------- */
+/* Pattern including the capacitor companion stamps (transient != 0). */
+int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** out);
+double orc_pulse_value(const double* p, double time);
+typedef struct orc_tran_opts {
+    double h;             /* step (s)                                     */
+    int32_t steps;
+    int32_t max_iter;     /* 50                                           */
+    int32_t start;        /* 0: zero state (D11), 1: v_io holds the state */
+    int32_t lu_kind;      /* ORC_LU_*                                     */
+} orc_tran_opts;
+typedef struct orc_tran_out {
+    int32_t iterations;   /* NR iterations over all steps                 */
+    int32_t status;       /* NR_STATUS_*                                  */
+    int32_t steps_done;
+    double kcl;           /* KCL of the last Newton iterate (A)           */
+} orc_tran_out;
+void orc_tran_opts_default(orc_tran_opts* o);
+/* wave (optional): [(steps+1) * n_probe], row 0 = start state, NaN where
+ * a step did not complete */
+int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_opts* opts, double* v_io,
+                        int32_t n_probe, const int32_t* probe, double* wave, orc_tran_out* out);
+int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                              const orc_tran_opts* opts, int32_t threads, double* v_io, int32_t n_probe,
+                              const int32_t* probe, double* wave, orc_tran_out* outs);
+
 int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
                          double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,
                          double* m_w, double* m_l, double* m_kp, double* m_vth, double* m_lam,

@@ -25,6 +25,15 @@ class OrcDcOpts(C.Structure):
                 ("start_step", C.c_int32), ("use_seed", C.c_int32)]
 
 
+class OrcTranOpts(C.Structure):
+    _fields_ = [("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
+                ("lu_kind", C.c_int32)]
+
+
+class OrcTranOut(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int32), ("kcl", C.c_double)]
+
+
 class OrcDcOut(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
                 ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
@@ -44,6 +53,12 @@ def lib():
             raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
         L = C.CDLL(path)
         L.orc_topology_build.argtypes = [C.POINTER(_abi.NrNetlist), C.POINTER(C.c_void_p)]
+        L.orc_topology_build_ex.argtypes = [C.POINTER(_abi.NrNetlist), C.c_int32, C.POINTER(C.c_void_p)]
+        L.orc_pulse_value.argtypes = [_f64p, C.c_double]
+        L.orc_pulse_value.restype = C.c_double
+        L.orc_solve_transient_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32,
+                                                C.POINTER(_f64p), C.POINTER(OrcTranOpts), C.c_int32, _f64p,
+                                                C.c_int32, _i32p, _f64p, C.POINTER(OrcTranOut)]
         L.orc_topology_free.argtypes = [C.c_void_p]
         L.orc_topology_info.argtypes = [C.c_void_p, _i32p]
         L.orc_topology_export.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64]
@@ -85,11 +100,11 @@ def _p32(a):
 class Topology:
     """Oracle topology: pattern, stamp map, Min-Fill ordering, fill pattern."""
 
-    def __init__(self, nl):
+    def __init__(self, nl, transient=False):
         self.nl = nl
         self._c, self._keep = nl.to_c()
         h = C.c_void_p()
-        rc = lib().orc_topology_build(C.byref(self._c), C.byref(h))
+        rc = lib().orc_topology_build_ex(C.byref(self._c), 1 if transient else 0, C.byref(h))
         if rc:
             raise RuntimeError(f"orc_topology_build failed: {rc}")
         self.h = h
@@ -152,6 +167,37 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL,
         "gmin_steps_done": np.array([x.gmin_steps_done for x in outs]),
     }
     return v, stats
+
+
+def pulse_value(p, t):
+    p = np.ascontiguousarray(p, np.float64)
+    return lib().orc_pulse_value(_p64(p), t)
+
+
+def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, steps=200, max_iter=50, start=0,
+                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0):
+    """Backward-Euler transient of B instances; returns (v_final [B, n],
+    wave [B, steps+1, n_probe], stats)."""
+    c, keep = nl.to_c()
+    arrs = (_f64p * _abi.N_FIELDS)()
+    hold = []
+    for fid in range(_abi.N_FIELDS):
+        if fid in fields:
+            a = np.ascontiguousarray(fields[fid][:B], dtype=np.float64)
+            hold.append(a)
+            arrs[fid] = _p64(a)
+        else:
+            arrs[fid] = None
+    v = np.zeros((B, nl.num_nodes)) if v0 is None else np.array(v0, dtype=np.float64).reshape(B, nl.num_nodes)
+    pr = np.ascontiguousarray(probes if probes is not None else [], np.int32)
+    wave = np.zeros((B, steps + 1, len(pr)))
+    outs = (OrcTranOut * B)()
+    o = OrcTranOpts(h, steps, max_iter, start, lu)
+    lib().orc_solve_transient_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), len(pr),
+                                    _p32(pr) if len(pr) else None, _p64(wave), outs)
+    stats = {"iterations": np.array([x.iterations for x in outs]), "status": np.array([x.status for x in outs]),
+             "steps_done": np.array([x.steps_done for x in outs]), "kcl": np.array([x.kcl for x in outs])}
+    return v, wave, stats
 
 
 def eval_diodes(a, c, Is, N, Vt, v):

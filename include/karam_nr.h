@@ -205,6 +205,9 @@ typedef struct nr_solve_opts {
     int32_t record_timing;   /* 1: per-phase HIP event timing; 2: also per
                                 kernel launch (nr_batch_get_kernel_stats)    */
     int32_t use_graph;       /* 1: replay each batched iteration as a hipGraph*/
+    int32_t fallback_plain;  /* 1: an instance whose seeded solve does not
+                                converge is solved again from the plain
+                                GMIN-stepping start                           */
 } nr_solve_opts;
 NR_API void nr_solve_opts_default(nr_solve_opts* opts);
 
@@ -276,6 +279,8 @@ typedef struct nr_instance_stats {
     int32_t status;              /* NR_STATUS_*                              */
     int32_t gmin_steps_done;
     double kcl;                  /* final max |KCL| (A), FP64                */
+    int32_t seeded;              /* 1: the result comes from the seed; 0: from
+                                    the plain start (no seed, or fallback)   */
 } nr_instance_stats;
 NR_API int nr_batch_get_stats(const nr_batch* batch, nr_instance_stats* out,
                               int64_t count);
@@ -315,6 +320,59 @@ typedef struct nr_kernel_stat {
 } nr_kernel_stat;
 NR_API int nr_batch_get_kernel_stats(const nr_batch* batch, nr_kernel_stat* out,
                                      int32_t capacity, int32_t* count);
+
+/* ---- learned seeder behind the NeuralSeeder seam ------------------------- */
+/* solvers/neural_seeder.h:45-220 exposes recordSolution / predictSeed /
+ * hasHistory / reset around an exponential-average blender (D20).  Here the
+ * same seam holds a multilayer perceptron: n_features -> n_hidden layers of
+ * `hidden` units with GELU -> n_out node voltages, BF16 weights and
+ * activations in the matrix products with FP32 accumulation, FP32 master
+ * weights trained with Adam on the GPU.  Features and voltages are
+ * normalised internally (per-feature and per-output mean and deviation of
+ * the recorded history). */
+typedef struct nr_seeder nr_seeder;
+typedef struct nr_seeder_cfg {
+    int32_t n_features;      /* multiple of 64                               */
+    int32_t n_out;           /* node voltages predicted (any size)           */
+    int32_t hidden;          /* multiple of 64                               */
+    int32_t n_hidden;        /* hidden layers, >= 1                          */
+    uint64_t init_seed;      /* weight initialisation                        */
+} nr_seeder_cfg;
+NR_API int nr_seeder_create(nr_ctx* ctx, const nr_seeder_cfg* cfg, nr_seeder** out);
+NR_API int nr_seeder_destroy(nr_seeder* s);
+/* recordSolution: count training pairs (features [count x n_features],
+ * voltages [count x n_out]) appended to the device-resident history */
+NR_API int nr_seeder_record(nr_seeder* s, const float* features, const double* v, int64_t count);
+/* hasHistory: number of recorded pairs */
+NR_API int64_t nr_seeder_history(const nr_seeder* s);
+/* reset: drops the history and re-initialises the weights */
+NR_API int nr_seeder_reset(nr_seeder* s);
+
+typedef struct nr_train_opts {
+    int32_t epochs;
+    int32_t batch;           /* minibatch rows, multiple of 64               */
+    float lr, beta1, beta2, eps;
+    uint64_t shuffle_seed;
+} nr_train_opts;
+NR_API void nr_train_opts_default(nr_train_opts* opts);
+/* Trains on the recorded history; *loss = mean squared error of the last
+ * epoch in normalised units. */
+NR_API int nr_seeder_train(nr_seeder* s, const nr_train_opts* opts, float* loss);
+/* predictSeed: voltages [count x n_out] for features [count x n_features] */
+NR_API int nr_seeder_predict(nr_seeder* s, const float* features, int64_t count, double* v_out);
+/* The batch's seed from the seeder, computed on the device (features
+ * [B x n_features]); the next nr_batch_solve_dc starts from it at
+ * seeded_start_step. */
+NR_API int nr_batch_set_seed_from(nr_batch* batch, nr_seeder* s, const float* features);
+/* Flat FP32 parameters: per layer W [in x out] row-major, then b [out]. */
+NR_API int64_t nr_seeder_param_count(const nr_seeder* s);
+NR_API int nr_seeder_get_params(const nr_seeder* s, float* out, int64_t count);
+NR_API int nr_seeder_set_params(nr_seeder* s, const float* in, int64_t count);
+/* One forward and backward pass without normalisation or update (gradient
+ * checks): x [count x n_features], y [count x n_out] in network units,
+ * grad = d(mean squared error)/d(params) in the flat layout. */
+NR_API int nr_seeder_loss_grad(nr_seeder* s, const float* x, const float* y, int64_t count, float* grad,
+                               float* loss);
 
 /* ---- KernelDispatcher slots (tensors/kernel_dispatch.h:24-31) ----------- */
 /* Drop-in for batchDiodePhysics_generic (tensors/physics_tensors.cpp:19-51):

@@ -98,6 +98,7 @@ class NrSolveOpts(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("precision", C.c_int32),
         ("seeded_start_step", C.c_int32), ("record_timing", C.c_int32), ("use_graph", C.c_int32),
+        ("fallback_plain", C.c_int32),
     ]
 
 
@@ -106,6 +107,16 @@ class NrTransientOpts(C.Structure):
         ("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
         ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)), ("record_timing", C.c_int32),
     ]
+
+
+class NrSeederCfg(C.Structure):
+    _fields_ = [("n_features", C.c_int32), ("n_out", C.c_int32), ("hidden", C.c_int32), ("n_hidden", C.c_int32),
+                ("init_seed", C.c_uint64)]
+
+
+class NrTrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("shuffle_seed", C.c_uint64)]
 
 
 class NrResult(C.Structure):
@@ -120,6 +131,7 @@ class NrInstanceStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
         ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
+        ("seeded", C.c_int32),
     ]
 
 

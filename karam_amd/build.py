@@ -17,7 +17,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libkaram_nr.so")
 SOURCES = [os.path.join(PKG, "csrc", "nr_device.hip"), os.path.join(PKG, "csrc", "topology.cpp")]
-DEPENDS = SOURCES + [os.path.join(PKG, "csrc", f) for f in ("nr_host.inc", "topology.h")] + \
+DEPENDS = SOURCES + [os.path.join(PKG, "csrc", f) for f in ("nr_host.inc", "seeder.inc", "topology.h")] + \
     [os.path.join(ROOT, "include", "karam_nr.h")]
 
 

@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <random>
 #include <string>
 #include <vector>
 
@@ -357,6 +358,53 @@ __device__ double pulse_value(const double* p, double time) {
 // kernels: set-up, evaluation, assembly
 // ---------------------------------------------------------------------------
 
+// seedInitialGuess (solvers/circuitsim_solver.cpp:32-116) into v, by a whole
+// block: V+ nodes at their source value, diode anodes at least 0.6 V, then
+// unset I-source + nodes at half the supply
+__device__ void seed_initial_guess(const DevTopo& T, const DevBatch& P, int b, double* v) {
+    const int n = T.n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double* V = P.vsrc + (size_t)b * T.nV;
+        for (int i = 0; i < T.nV; ++i) {
+            int p = T.vsrc_pos[i];
+            if (p > 0 && p <= n) v[p - 1] = V[i];
+        }
+        for (int i = 0; i < T.nD; ++i) {   // netlist order
+            int a = T.dio_anode_nl[i];
+            if (a > 0 && a <= n) v[a - 1] = ref_max(v[a - 1], 0.6);
+        }
+        double vs = 5.0;
+        for (int i = 0; i < T.nV; ++i)
+            if (V[i] > vs) vs = V[i];
+        for (int i = 0; i < T.nI; ++i) {
+            int p = T.isrc_pos[i];
+            if (p <= 0 || p > n) continue;
+            if (v[p - 1] == 0.0) v[p - 1] = vs * 0.5;
+        }
+    }
+    __syncthreads();
+}
+
+// Plain-start fallback (one block per instance): an instance whose seeded
+// solve did not converge restarts from seedInitialGuess at the first GMIN
+// step; its iteration counts keep accumulating.
+__global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P) {
+    const int b = blockIdx.x;
+    InstState st = P.st[b];
+    if (st.status == NR_STATUS_CONVERGED) return;
+    double* v = P.v + (size_t)b * T.n;
+    seed_initial_guess(T, P, b, v);
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.vI[IX(i, b, P.Bp)] = v[i];
+    if (threadIdx.x == 0) {
+        st.step = 0; st.it = 0;
+        st.status = NR_STATUS_NOT_CONVERGED; st.done = 0; st.singular = 0; st.seeded = 0;
+        st.prev_kcl = 0.0; st.kcl = 0.0;
+        P.st[b] = st;
+    }
+}
+
 // Start state of one instance (one block per instance) and the per-instance
 // linear part of A and rhs.  mode 0 (DC): seedInitialGuess
 // (solvers/circuitsim_solver.cpp:32-116) or the caller's seed; mode 1
@@ -384,27 +432,7 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     } else if (mode != 0) {
         for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = 0.0;
     } else {
-        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = 0.0;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const double* V = P.vsrc + (size_t)b * T.nV;
-            for (int i = 0; i < T.nV; ++i) {
-                int p = T.vsrc_pos[i];
-                if (p > 0 && p <= n) v[p - 1] = V[i];
-            }
-            for (int i = 0; i < T.nD; ++i) {   // netlist order
-                int a = T.dio_anode_nl[i];
-                if (a > 0 && a <= n) v[a - 1] = ref_max(v[a - 1], 0.6);
-            }
-            double vs = 5.0;
-            for (int i = 0; i < T.nV; ++i)
-                if (V[i] > vs) vs = V[i];
-            for (int i = 0; i < T.nI; ++i) {
-                int p = T.isrc_pos[i];
-                if (p <= 0 || p > n) continue;
-                if (v[p - 1] == 0.0) v[p - 1] = vs * 0.5;
-            }
-        }
+        seed_initial_guess(T, P, b, v);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) P.vI[IX(i, b, Bp)] = v[i];
@@ -1191,3 +1219,4 @@ __global__ void k_eval_mos_slot(int count, const int* d, const int* g, const int
 }
 
 #include "nr_host.inc"
+#include "seeder.inc"

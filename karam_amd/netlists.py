@@ -254,3 +254,37 @@ def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: f
         out[_abi.NR_FIELD_MOS_KP] = KP
         out[_abi.NR_FIELD_MOS_VTO] = VT
     return out
+
+
+def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
+    """Seeder input features of Monte-Carlo instances, [B, n_features] FP32:
+    the per-device deviations from the template netlist (log ratios for R,
+    IS, KP, capacitance and source values; differences for VTO), in netlist
+    order and field order, averaged over n_features contiguous blocks."""
+    kp0, vto0 = nl.card_params()
+    nominal = {
+        _abi.NR_FIELD_RES_OHMS: (nl.res_ohms, "log"), _abi.NR_FIELD_DIO_IS: (nl.dio_is, "log"),
+        _abi.NR_FIELD_MOS_KP: (kp0, "log"), _abi.NR_FIELD_MOS_VTO: (vto0, "diff"),
+        _abi.NR_FIELD_ISRC_AMPS: (nl.isrc_amps, "log"), _abi.NR_FIELD_VSRC_VOLTS: (nl.vsrc_volts, "diff"),
+        _abi.NR_FIELD_CAP_FARADS: (nl.cap_farads, "log"),
+    }
+    cols = []
+    for fid in sorted(fields):
+        x = np.asarray(fields[fid], dtype=np.float64)
+        base, kind = nominal[fid]
+        if kind == "log":
+            with np.errstate(divide="ignore", invalid="ignore"):
+                d = np.log(np.abs(x) / np.abs(base))
+            d[~np.isfinite(d)] = 0.0
+        else:
+            d = x - base
+        cols.append(d)
+    if not cols:
+        raise ValueError("no perturbed fields")
+    D = np.concatenate(cols, axis=1)
+    blocks = np.array_split(np.arange(D.shape[1]), n_features)
+    out = np.zeros((D.shape[0], n_features), dtype=np.float32)
+    for k, idx in enumerate(blocks):
+        if len(idx):
+            out[:, k] = D[:, idx].mean(axis=1)
+    return out

@@ -40,6 +40,9 @@ EXPORTED = [
     "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
     "nr_eval_diodes", "nr_eval_mosfets",
     "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform",
+    "nr_seeder_create", "nr_seeder_destroy", "nr_seeder_record", "nr_seeder_history", "nr_seeder_reset",
+    "nr_train_opts_default", "nr_seeder_train", "nr_seeder_predict", "nr_batch_set_seed_from",
+    "nr_seeder_param_count", "nr_seeder_get_params", "nr_seeder_set_params", "nr_seeder_loss_grad",
 ]
 
 
@@ -64,6 +67,23 @@ def lib():
         L.nr_transient_opts_default.restype = None
         L.nr_batch_transient.argtypes = [_vp, C.POINTER(_abi.NrTransientOpts), C.POINTER(_abi.NrResult)]
         L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
+        _f32p = C.POINTER(C.c_float)
+        L.nr_seeder_create.argtypes = [_vp, C.POINTER(_abi.NrSeederCfg), C.POINTER(_vp)]
+        L.nr_seeder_destroy.argtypes = [_vp]
+        L.nr_seeder_record.argtypes = [_vp, _f32p, _f64p, C.c_int64]
+        L.nr_seeder_history.argtypes = [_vp]
+        L.nr_seeder_history.restype = C.c_int64
+        L.nr_seeder_reset.argtypes = [_vp]
+        L.nr_train_opts_default.argtypes = [C.POINTER(_abi.NrTrainOpts)]
+        L.nr_train_opts_default.restype = None
+        L.nr_seeder_train.argtypes = [_vp, C.POINTER(_abi.NrTrainOpts), _f32p]
+        L.nr_seeder_predict.argtypes = [_vp, _f32p, C.c_int64, _f64p]
+        L.nr_batch_set_seed_from.argtypes = [_vp, _vp, _f32p]
+        L.nr_seeder_param_count.argtypes = [_vp]
+        L.nr_seeder_param_count.restype = C.c_int64
+        L.nr_seeder_get_params.argtypes = [_vp, _f32p, C.c_int64]
+        L.nr_seeder_set_params.argtypes = [_vp, _f32p, C.c_int64]
+        L.nr_seeder_loss_grad.argtypes = [_vp, _f32p, _f32p, C.c_int64, _f32p, _f32p]
         L.nr_topology_destroy.argtypes = [_vp]
         L.nr_topology_get_info.argtypes = [_vp, C.POINTER(_abi.NrTopologyInfo)]
         L.nr_topology_export.argtypes = [_vp, C.c_int32, _i32p, C.c_int64, C.POINTER(C.c_int64)]
@@ -100,6 +120,82 @@ def _p64(a):
 
 def _p32(a):
     return a.ctypes.data_as(_i32p)
+
+
+def _p32f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Seeder:
+    """The learned seeder behind the NeuralSeeder seam
+    (solvers/neural_seeder.h:45-220): record_solution / predict_seed /
+    has_history / reset, plus train().  An MLP n_features -> n_hidden x
+    hidden (GELU) -> n_out trained and evaluated on the GPU."""
+
+    def __init__(self, ctx: Context, n_features: int, n_out: int, hidden: int = 1024, n_hidden: int = 3,
+                 init_seed: int = 1):
+        self.ctx, self.n_features, self.n_out = ctx, int(n_features), int(n_out)
+        cfg = _abi.NrSeederCfg(self.n_features, self.n_out, int(hidden), int(n_hidden), int(init_seed))
+        h = _vp()
+        _check(lib().nr_seeder_create(ctx.h, C.byref(cfg), C.byref(h)), ctx)
+        self.h = h
+
+    def record_solution(self, features, v):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, self.n_features)
+        vv = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, self.n_out)
+        _check(lib().nr_seeder_record(self.h, _p32f(f), _p64(vv), f.shape[0]), self.ctx)
+
+    def history(self) -> int:
+        return int(lib().nr_seeder_history(self.h))
+
+    def has_history(self) -> bool:
+        return self.history() > 0
+
+    def reset(self):
+        _check(lib().nr_seeder_reset(self.h), self.ctx)
+
+    def train(self, epochs=40, batch=256, lr=1e-3, shuffle_seed=1):
+        o = _abi.NrTrainOpts()
+        lib().nr_train_opts_default(C.byref(o))
+        o.epochs, o.batch, o.lr, o.shuffle_seed = int(epochs), int(batch), float(lr), int(shuffle_seed)
+        loss = C.c_float()
+        _check(lib().nr_seeder_train(self.h, C.byref(o), C.byref(loss)), self.ctx)
+        return float(loss.value)
+
+    def predict_seed(self, features):
+        f = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, self.n_features)
+        out = np.zeros((f.shape[0], self.n_out))
+        _check(lib().nr_seeder_predict(self.h, _p32f(f), f.shape[0], _p64(out)), self.ctx)
+        return out
+
+    def params(self):
+        n = int(lib().nr_seeder_param_count(self.h))
+        out = np.zeros(n, dtype=np.float32)
+        _check(lib().nr_seeder_get_params(self.h, _p32f(out), n), self.ctx)
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().nr_seeder_set_params(self.h, _p32f(p), p.size), self.ctx)
+
+    def loss_grad(self, x, y):
+        """Loss and flat gradient of one forward/backward pass (network units)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.ascontiguousarray(y, dtype=np.float32)
+        n = int(lib().nr_seeder_param_count(self.h))
+        g = np.zeros(n, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().nr_seeder_loss_grad(self.h, _p32f(x), _p32f(y), x.shape[0], _p32f(g), C.byref(loss)),
+               self.ctx)
+        return float(loss.value), g
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nr_seeder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class Context:
@@ -152,11 +248,12 @@ class Topology:
 
 
 def solve_opts(max_iter=50, gmin_steps=19, precision=_abi.NR_PREC_FP64, seeded_start_step=18,
-               record_timing=0, use_graph=0):
+               record_timing=0, use_graph=0, fallback_plain=1):
     o = _abi.NrSolveOpts()
     lib().nr_solve_opts_default(C.byref(o))
     o.max_iter, o.gmin_steps, o.precision = max_iter, gmin_steps, precision
     o.seeded_start_step, o.record_timing, o.use_graph = seeded_start_step, record_timing, use_graph
+    o.fallback_plain = fallback_plain
     return o
 
 
@@ -180,6 +277,11 @@ class Batch:
     def set_seed(self, v0):
         a = np.ascontiguousarray(v0, dtype=np.float64)
         _check(lib().nr_batch_set_seed(self.h, _p64(a), a.size), self.ctx)
+
+    def set_seed_from(self, seeder: "Seeder", features):
+        """The seed of every instance predicted by the seeder on the device."""
+        f = np.ascontiguousarray(features, dtype=np.float32)
+        _check(lib().nr_batch_set_seed_from(self.h, seeder.h, _p32f(f)), self.ctx)
 
     def clear_seed(self):
         _check(lib().nr_batch_clear_seed(self.h), self.ctx)
@@ -227,6 +329,7 @@ class Batch:
             "status": np.array([x.status for x in arr]),
             "gmin_steps_done": np.array([x.gmin_steps_done for x in arr]),
             "kcl": np.array([x.kcl for x in arr]),
+            "seeded": np.array([x.seeded for x in arr]),
         }
 
     def timing(self):

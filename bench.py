@@ -63,6 +63,23 @@ def load_traffic(kernel):
     return None
 
 
+def seeded_parity(nl, fields, sd):
+    """The oracle started from the GPU's seeds on the first instances."""
+    from karam_amd import _abi, netlists
+    from oracle import oracle as O
+    T = O.Topology(nl)
+    errs, same_it = [], 0
+    for b in range(len(sd["_seed_sample"])):
+        inst = netlists.Netlist(**{**nl.__dict__})
+        inst.res_ohms = fields[_abi.NR_FIELD_RES_OHMS][b]
+        inst.dio_is = fields[_abi.NR_FIELD_DIO_IS][b]
+        vo, out = O.solve_dc(inst, T, seed_v=sd["_seed_sample"][b], start_step=18)
+        errs.append(float((np.abs(sd["_v_sample"][b] - vo) / (1e-9 * np.abs(vo) + 1e-12)).max()))
+        same_it += int(out.iterations == sd["_it_sample"][b])
+    return {"instances": len(errs), "max_err_over_tol": max(errs), "iterations_equal": same_it,
+            "reference": "oracle started from the same seed at the last GMIN step"}
+
+
 def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
     """Oracle over a bounded sample of the same instances on `threads` host
     threads; returns (baseline dict, parity dict)."""
@@ -94,6 +111,70 @@ def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
     return base, parity
 
 
+SEEDER_FEATURES, SEEDER_HIDDEN, SEEDER_LAYERS = 512, 1024, 3
+
+
+def seeded_path(a, ctx, nl, T, bt, fields, B, it_plain):
+    """The learned-seeder path on the same instances: train on GPU-solved
+    instances drawn from a disjoint seed range (untimed, reported), then time
+    `steps` solves that each predict every instance's seed on the device and
+    start Newton at the last GMIN step, with the plain start as fallback."""
+    from karam_amd import _abi, netlists, nr
+    n = T.info["n"]
+    t0 = time.perf_counter()
+    s = nr.Seeder(ctx, SEEDER_FEATURES, n, hidden=SEEDER_HIDDEN, n_hidden=SEEDER_LAYERS, init_seed=1)
+    gen = nr.Batch(ctx, T, B)
+    done = 0
+    while done < a.seeder_train:
+        m = min(B, a.seeder_train - done)
+        ftr = netlists.mc_fields(nl, B, seed=500000 + done)
+        gen.set_fields(ftr)
+        gen.solve_dc()
+        st = gen.stats()
+        ok = st["status"] == _abi.NR_STATUS_CONVERGED
+        ok[m:] = False
+        s.record_solution(netlists.mc_features(nl, ftr, SEEDER_FEATURES)[ok], gen.voltages()[ok])
+        done += m
+    gen.close()
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss = s.train(epochs=a.seeder_epochs, batch=256, lr=1e-3)
+    t_train = time.perf_counter() - t0
+    log(f"seeder: {s.history()} solutions in {t_data:.1f} s, trained in {t_train:.1f} s, loss {loss:.4f}")
+    X = netlists.mc_features(nl, fields, SEEDER_FEATURES)
+    bt.set_seed_from(s, X)
+    bt.solve_dc(seeded_start_step=18)                         # warm-up
+    conv, t_pred, sweeps = 0, 0.0, 0
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        tp = time.perf_counter()
+        bt.set_seed_from(s, X)
+        t_pred += time.perf_counter() - tp
+        r = bt.solve_dc(seeded_start_step=18)
+        st = bt.stats()
+        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+        sweeps = r["batched_iterations"]
+        log(f"seeded step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
+    total = time.perf_counter() - t0
+    it = st["iterations"]
+    dims = [SEEDER_FEATURES] + [SEEDER_HIDDEN] * SEEDER_LAYERS + [(n + 63) // 64 * 64]
+    flops = 2.0 * B * sum(x * y for x, y in zip(dims[:-1], dims[1:]))
+    seed_v = s.predict_seed(X[:8])
+    return {
+        "value": conv / total, "unit": "solves/s", "ms_per_step": total * 1e3 / a.steps,
+        "speedup_vs_plain": None, "batched_sweeps_per_step": sweeps,
+        "fallback_instances": int((st["seeded"] == 0).sum()),
+        "iterations": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
+                       "p99": float(np.percentile(it, 99))},
+        "seeder": {"features": SEEDER_FEATURES, "hidden": SEEDER_HIDDEN, "layers": SEEDER_LAYERS, "outputs": n,
+                   "train_instances": int(s.history()), "train_data_s": t_data, "train_s": t_train,
+                   "final_loss_normalised": loss, "predict_ms_per_step": t_pred * 1e3 / a.steps,
+                   "predict_tflops_wall": flops / (t_pred / a.steps) / 1e12,
+                   "mfma_peak_tflops_bf16": 2500.0},
+        "_seed_sample": seed_v, "_v_sample": bt.voltages()[:8], "_it_sample": it[:8],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +185,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-seeder", action="store_true")
+    ap.add_argument("--seeder-train", type=int, default=8192)
+    ap.add_argument("--seeder-epochs", type=int, default=20)
     ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.copy_probe:
@@ -178,12 +262,22 @@ def main():
                     sorted(kst.items(), key=lambda kv: -kv[1]["ms"])},
         "cpu_baseline": None,
     }
+    if not a.no_seeder:
+        out["seeded"] = seeded_path(a, ctx, nl, T, bt, fields, B, it)
+        out["nr_iters_per_solve"]["seeded"] = out["seeded"].pop("iterations")
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         base, parity = cpu_baseline(nl, fields, B, V, it, a.cpu_seconds, threads)
         out["cpu_baseline"] = base
         out["parity_sample"] = parity
+        if "seeded" in out:
+            out["seeded"]["parity_sample"] = seeded_parity(nl, fields, out["seeded"])
         log(f"cpu baseline {base['value']:.1f} solves/s on {threads} threads; parity {parity}")
+    if "seeded" in out:
+        sd = out["seeded"]
+        sd["speedup_vs_plain"] = sd["value"] / out["value"] if out["value"] else None
+        for k in [k for k in sd if k.startswith("_")]:
+            del sd[k]
     print(json.dumps(out), flush=True)
     return 0
 

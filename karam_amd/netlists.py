@@ -288,3 +288,87 @@ def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
         if len(idx):
             out[:, k] = D[:, idx].mean(axis=1)
     return out
+
+
+def random_logic(n_target: int = 20000, seed: int = 1, n_inputs: int = 32, window: int = 3, width: int = 32,
+                 spread: int = 2, p_inv: float = 0.3, max_fanout: int = 4, vdd: float = 1.8,
+                 cload: float = 10e-15, n_pulse: int = 8) -> Netlist:
+    """Levelised random CMOS logic of NAND2 and INV gates (level-1 MOSFETs,
+    the inverter-chain cards).  Node 1 is VDD; then n_inputs primary inputs
+    driven by DC sources at 0 or VDD (seeded); every gate output has a
+    `cload` capacitor to ground.  Gate inputs are drawn from the nets of the
+    previous `window` levels (locality), each net driving at most
+    `max_fanout` gates.  A NAND2 is two series NMOS (with an internal node)
+    and two parallel PMOS; an INV is one NMOS and one PMOS.  Gates are added
+    until the node count reaches n_target.  Each level has `width` gate
+    slots; slot p draws its inputs from slots p - spread .. p + spread of the
+    previous `window` levels, which keeps the circuit banded (its
+    elimination fronts grow with the width, not with the size).  The first
+    n_pulse primary inputs are PULSE sources (0 -> VDD, seeded delay, width
+    and period; DC value 0) so that a transient has switching activity."""
+    rng = np.random.default_rng(seed)
+    md, mg, ms, mb, mw, ml, mm = [], [], [], [], [], [], []
+    cap_n = []
+    node = 1                                   # VDD
+    pis = list(range(2, 2 + n_inputs))
+    node = 1 + n_inputs
+    fan = {p: 0 for p in pis}
+    levels = [pis]
+
+    def nmos(d, g, s):
+        md.append(d); mg.append(g); ms.append(s); mb.append(0); mw.append(2e-6); ml.append(1e-6); mm.append(0)
+
+    def pmos(d, g, s):
+        md.append(d); mg.append(g); ms.append(s); mb.append(1); mw.append(4e-6); ml.append(1e-6); mm.append(1)
+
+    def near(lv, p):
+        scale = len(lv) / float(width)
+        c = int(p * scale)
+        lo, hi = max(0, c - spread), min(len(lv), c + spread + 1)
+        return lv[lo:hi]
+
+    def pick(k, p):
+        cand = [x for lv in levels[-window:] for x in near(lv, p)]
+        pool = [x for x in cand if fan[x] < max_fanout]
+        if len(pool) < k:
+            pool = cand
+        return list(rng.choice(pool, size=k, replace=False))
+
+    while node < n_target:
+        cur = []
+        for p in range(width):
+            if node >= n_target:
+                break
+            if rng.random() < p_inv:
+                (a,) = pick(1, p)
+                out = node + 1; node = out
+                nmos(out, a, 0); pmos(out, a, 1)
+                fan[a] += 1
+            else:
+                a, b = pick(2, p)
+                out, mid = node + 1, node + 2; node = mid
+                nmos(out, a, mid); nmos(mid, b, 0)
+                pmos(out, a, 1); pmos(out, b, 1)
+                fan[a] += 1; fan[b] += 1
+            fan[out] = 0
+            cur.append(out)
+            cap_n.append(out)
+        levels.append(cur)
+    pi_v = rng.integers(0, 2, n_inputs) * vdd
+    n_pulse = min(n_pulse, n_inputs)
+    pulse = np.zeros((1 + n_inputs, 7))
+    is_pulse = np.zeros(1 + n_inputs, dtype=np.int32)
+    for k in range(n_pulse):
+        per = rng.uniform(20e-9, 60e-9)
+        pulse[1 + k] = [0.0, vdd, rng.uniform(1e-9, 10e-9), 0.5e-9, 0.5e-9, rng.uniform(0.3, 0.6) * per, per]
+        is_pulse[1 + k] = 1
+        pi_v[k] = 0.0
+    return Netlist(
+        num_nodes=node,
+        cap_n1=_i32(cap_n), cap_n2=_i32(np.zeros(len(cap_n))), cap_farads=_f64(np.full(len(cap_n), cload)),
+        vsrc_pos=_i32([1] + pis), vsrc_neg=_i32(np.zeros(1 + n_inputs)), vsrc_volts=_f64([vdd] + list(pi_v)),
+        vsrc_is_pulse=_i32(is_pulse), vsrc_pulse=_f64(pulse),
+        mos_d=_i32(md), mos_g=_i32(mg), mos_s=_i32(ms), mos_b=_i32(mb),
+        mos_w=_f64(mw), mos_l=_f64(ml), mos_model=_i32(mm),
+        models=[NMOS_CARD, PMOS_CARD], name=f"logic{node}",
+    )

@@ -175,6 +175,68 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, it_plain):
     }
 
 
+def logic_workload(a):
+    """Random CMOS logic (levelised inverter DAG with switching PULSE
+    inputs, level-1 MOSFETs), 2048 Monte-Carlo instances (VTO sigma 30 mV,
+    KP sigma 3 %): DC operating points, then a fixed-step backward-Euler
+    transient from them.  The default size (2000 nodes) is the largest at
+    which the reference's Newton method converges on this family."""
+    from karam_amd import _abi, netlists, nr
+    B = a.instances if a.instances != 4096 else 2048
+    nl = netlists.random_logic(a.logic_nodes, seed=2, p_inv=1.0)
+    f = netlists.mc_fields(nl, B, seed=1000, sigma_vto=0.03, sigma_kp=0.03)
+    ctx = nr.Context(0)
+    t = time.perf_counter()
+    T = nr.Topology(nl, ctx)
+    Tt = nr.Topology(nl, ctx, transient=True)
+    symbolic_ms = (time.perf_counter() - t) * 1e3 / 2
+    log(f"logic n={T.info['n']} mos={T.info['n_mos']} nnz_f={T.info['nnz_f']} tw={T.info['treewidth']} "
+        f"levels={T.info['n_levels']} max_front={T.info['max_front']}")
+    bt = nr.Batch(ctx, T, B)
+    bt.set_fields(f)
+    for _ in range(a.warmup):
+        bt.solve_dc()
+    conv = 0
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        r = bt.solve_dc()
+        st = bt.stats()
+        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+        log(f"dc step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {r['batched_iterations']}")
+    total = time.perf_counter() - t0
+    it = st["iterations"]
+    V = bt.voltages()
+    bt.close()
+    tr = nr.Batch(ctx, Tt, B)
+    tr.set_fields(f)
+    tr.set_seed(V)
+    t0 = time.perf_counter()
+    rt = tr.transient(h=0.5e-9, steps=a.tran_steps, start=_abi.NR_TRAN_START_SEED,
+                      probes=[2, 3, T.info["n"] // 2, T.info["n"]])
+    t_tran = time.perf_counter() - t0
+    stt = tr.stats()
+    out = {
+        "metric": METRIC, "value": conv / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": total * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded levelised random inverter logic, per-device VTO sigma 30 mV and KP sigma 3 %",
+        "config": {"workload": f"logic{T.info['n']} Monte-Carlo DC operating point + 0.5 ns backward-Euler "
+                               f"transient",
+                   "instances": B, "nodes": T.info["n"], "mosfets": T.info["n_mos"], "nnz_f": T.info["nnz_f"],
+                   "treewidth": T.info["treewidth"], "symbolic_ms": symbolic_ms},
+        "nr_iters_per_solve": {"plain": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
+                                         "p99": float(np.percentile(it, 99))}, "seeded": None},
+        "transient": {"steps": a.tran_steps, "h_s": 0.5e-9, "start": "DC operating point",
+                      "instances_completed": int(rt["n_converged"]), "wall_s": t_tran,
+                      "instance_steps_per_s": rt["n_converged"] * a.tran_steps / t_tran,
+                      "newton_iterations_per_step": float(stt["iterations"].mean() / a.tran_steps),
+                      "batched_sweeps": rt["batched_iterations"]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -188,6 +250,9 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
+    ap.add_argument("--workload", choices=["rdmesh64", "logic"], default="rdmesh64")
+    ap.add_argument("--logic-nodes", type=int, default=2000)
+    ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--copy-probe", action="store_true")
     a = ap.parse_args()
     if a.copy_probe:
@@ -196,6 +261,9 @@ def main():
         print("bench.py measures one GPU: Monte-Carlo instances shard by contiguous ranges with no exchange, "
               "and multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)
         return 2
+
+    if a.workload == "logic":
+        return logic_workload(a)
 
     from karam_amd import _abi, netlists, nr
 

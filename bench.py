@@ -4,9 +4,15 @@ One step = one batched DC solve of all instances to convergence (19-step
 GMIN plan, solveDC semantics), with netlist parameters already resident in
 HBM.  Prints ONE JSON line:
 
-  value        converged solves/s (FP64 KCL <= 1e-9 A), whole job, 1 GPU
+  value        converged solves/s (FP64 KCL <= 1e-9 A), whole job, 1 GPU,
+               plain FP64 path (the reference's method and arithmetic)
   roofline     the kernel with the largest device time in the timed steps:
                algorithmic bytes per launch / its mean HIP-event launch time
+  fp32_refine  the same solves on the FP32 path (FP32 Jacobian and LU, FP64
+               Newton residual and update), with its error against the FP64
+               results
+  seeded       learned-seeder starts on the FP64 and on the FP32 path; the
+               FP32 one's speedup over plain FP64 is `target_speedup`
   cpu_baseline the oracle (FP64 C restatement, OpenMP over instances) timed
                on this host's cores over a bounded sample of the instances,
                whose voltages are also compared node by node with the GPU's
@@ -63,21 +69,16 @@ def load_traffic(kernel):
     return None
 
 
-def seeded_parity(nl, fields, sd):
-    """The oracle started from the GPU's seeds on the first instances."""
-    from karam_amd import _abi, netlists
+def seeded_parity(nl, fields, seeds, leg):
+    """The FP64 oracle started from the GPU's seeds on the first instances,
+    against one seeded leg's voltages and iteration counts."""
     from oracle import oracle as O
-    T = O.Topology(nl)
-    errs, same_it = [], 0
-    for b in range(len(sd["_seed_sample"])):
-        inst = netlists.Netlist(**{**nl.__dict__})
-        inst.res_ohms = fields[_abi.NR_FIELD_RES_OHMS][b]
-        inst.dio_is = fields[_abi.NR_FIELD_DIO_IS][b]
-        vo, out = O.solve_dc(inst, T, seed_v=sd["_seed_sample"][b], start_step=18)
-        errs.append(float((np.abs(sd["_v_sample"][b] - vo) / (1e-9 * np.abs(vo) + 1e-12)).max()))
-        same_it += int(out.iterations == sd["_it_sample"][b])
-    return {"instances": len(errs), "max_err_over_tol": max(errs), "iterations_equal": same_it,
-            "reference": "oracle started from the same seed at the last GMIN step"}
+    k = len(seeds)
+    Vo, so = O.solve_dc_batch(nl, O.Topology(nl), {f: v[:k] for f, v in fields.items()}, k, threads=8,
+                              seeds=seeds, start_step=18)
+    return {"instances": k, "max_err_over_tol": err_over_tol(leg["_v_sample"], Vo),
+            "iterations_equal": int((leg["_it_sample"] == so["iterations"]).sum()),
+            "reference": "FP64 oracle started from the same seed at the last GMIN step"}
 
 
 def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
@@ -114,11 +115,59 @@ def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
 SEEDER_FEATURES, SEEDER_HIDDEN, SEEDER_LAYERS = 512, 1024, 3
 
 
-def seeded_path(a, ctx, nl, T, bt, fields, B, it_plain):
+def timed_solves(bt, steps, **kw):
+    """`steps` batched DC solves with per-kernel accounting; returns the
+    converged-solve rate, ms per solve, summed kernel rows and the last
+    solve's statistics and result."""
+    from karam_amd import _abi
+    kst, conv = {}, 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        r = bt.solve_dc(record_timing=2, **kw)
+        st = bt.stats()
+        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+        for name, x in bt.kernel_stats().items():
+            acc = kst.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            for f in acc:
+                acc[f] += x[f]
+        log(f"step {k} {kw}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {r['batched_iterations']}")
+    total = time.perf_counter() - t0
+    return conv / total, total * 1e3 / steps, kst, st, r
+
+
+def roofline(kst, traffic=True):
+    dom = max(kst, key=lambda n: kst[n]["ms"])
+    d = kst[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": load_traffic(dom) if traffic else None, "kernel": dom,
+            "algorithmic_bytes_per_launch": d["bytes"] / d["launches"],
+            "mean_launch_us": d["ms"] * 1e3 / d["launches"], "launches": d["launches"],
+            "device_ms_share": d["ms"] / sum(x["ms"] for x in kst.values())}
+
+
+def kernel_table(kst):
+    return {n: {"launches": x["launches"], "ms": round(x["ms"], 3),
+                "GBps": round(x["bytes"] / max(x["ms"], 1e-9) / 1e6, 1)} for n, x in
+            sorted(kst.items(), key=lambda kv: -kv[1]["ms"])}
+
+
+def err_over_tol(v, ref):
+    """max |v - ref| / (1e-9 |ref| + 1e-12 V)"""
+    return float((np.abs(v - ref) / (1e-9 * np.abs(ref) + 1e-12)).max())
+
+
+def iter_stats(it):
+    return {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)), "p99": float(np.percentile(it, 99))}
+
+
+def seeded_path(a, ctx, nl, T, bt, fields, B, V64):
     """The learned-seeder path on the same instances: train on GPU-solved
     instances drawn from a disjoint seed range (untimed, reported), then time
     `steps` solves that each predict every instance's seed on the device and
-    start Newton at the last GMIN step, with the plain start as fallback."""
+    start Newton at the last GMIN step, with the plain start as fallback, on
+    the FP64 and on the FP32 path."""
     from karam_amd import _abi, netlists, nr
     n = T.info["n"]
     t0 = time.perf_counter()
@@ -142,37 +191,42 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, it_plain):
     t_train = time.perf_counter() - t0
     log(f"seeder: {s.history()} solutions in {t_data:.1f} s, trained in {t_train:.1f} s, loss {loss:.4f}")
     X = netlists.mc_features(nl, fields, SEEDER_FEATURES)
-    bt.set_seed_from(s, X)
-    bt.solve_dc(seeded_start_step=18)                         # warm-up
-    conv, t_pred, sweeps = 0, 0.0, 0
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        tp = time.perf_counter()
-        bt.set_seed_from(s, X)
-        t_pred += time.perf_counter() - tp
-        r = bt.solve_dc(seeded_start_step=18)
-        st = bt.stats()
-        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
-        sweeps = r["batched_iterations"]
-        log(f"seeded step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
-    total = time.perf_counter() - t0
-    it = st["iterations"]
+    seed_v = s.predict_seed(X[:8])
     dims = [SEEDER_FEATURES] + [SEEDER_HIDDEN] * SEEDER_LAYERS + [(n + 63) // 64 * 64]
     flops = 2.0 * B * sum(x * y for x, y in zip(dims[:-1], dims[1:]))
-    seed_v = s.predict_seed(X[:8])
-    return {
-        "value": conv / total, "unit": "solves/s", "ms_per_step": total * 1e3 / a.steps,
-        "speedup_vs_plain": None, "batched_sweeps_per_step": sweeps,
-        "fallback_instances": int((st["seeded"] == 0).sum()),
-        "iterations": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
-                       "p99": float(np.percentile(it, 99))},
-        "seeder": {"features": SEEDER_FEATURES, "hidden": SEEDER_HIDDEN, "layers": SEEDER_LAYERS, "outputs": n,
-                   "train_instances": int(s.history()), "train_data_s": t_data, "train_s": t_train,
-                   "final_loss_normalised": loss, "predict_ms_per_step": t_pred * 1e3 / a.steps,
-                   "predict_tflops_wall": flops / (t_pred / a.steps) / 1e12,
-                   "mfma_peak_tflops_bf16": 2500.0},
-        "_seed_sample": seed_v, "_v_sample": bt.voltages()[:8], "_it_sample": it[:8],
-    }
+    out = {}
+    for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
+        bt.set_seed_from(s, X)
+        bt.solve_dc(seeded_start_step=18, precision=prec)         # warm-up
+        conv, t_pred, sweeps = 0, 0.0, 0
+        t0 = time.perf_counter()
+        for k in range(a.steps):
+            tp = time.perf_counter()
+            bt.set_seed_from(s, X)
+            t_pred += time.perf_counter() - tp
+            r = bt.solve_dc(seeded_start_step=18, precision=prec)
+            st = bt.stats()
+            conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+            sweeps = r["batched_iterations"]
+            log(f"seeded {key} step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
+        total = time.perf_counter() - t0
+        V = bt.voltages()
+        out[key] = {
+            "value": conv / total, "unit": "solves/s", "ms_per_step": total * 1e3 / a.steps,
+            "speedup_vs_plain": None, "batched_sweeps_per_step": sweeps,
+            "fallback_instances": int((st["seeded"] == 0).sum()),
+            "iterations": iter_stats(st["iterations"]),
+            "max_err_over_tol_vs_plain_fp64": err_over_tol(V, V64),
+            "predict_ms_per_step": t_pred * 1e3 / a.steps,
+            "_v_sample": V[:8], "_it_sample": st["iterations"][:8],
+        }
+    out["seeder"] = {"features": SEEDER_FEATURES, "hidden": SEEDER_HIDDEN, "layers": SEEDER_LAYERS, "outputs": n,
+                     "train_instances": int(s.history()), "train_data_s": t_data, "train_s": t_train,
+                     "final_loss_normalised": loss,
+                     "predict_tflops_wall": flops / (out["fp64"]["predict_ms_per_step"] * 1e-3) / 1e12,
+                     "mfma_peak_tflops_bf16": 2500.0}
+    out["_seed_sample"] = seed_v
+    return out
 
 
 def logic_workload(a):
@@ -281,71 +335,63 @@ def main():
         r = bt.solve_dc()
         log(f"warmup {w}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
 
-    kst = {}
-    conv = 0
-    sweeps = []
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        r = bt.solve_dc(record_timing=2)
-        st = bt.stats()
-        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
-        sweeps.append(r["batched_iterations"])
-        for name, x in bt.kernel_stats().items():
-            acc = kst.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
-            for f in acc:
-                acc[f] += x[f]
-        log(f"step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
-    total = time.perf_counter() - t0
+    rate, ms, kst, st, r = timed_solves(bt, a.steps)
     V = bt.voltages()
-    st = bt.stats()
-    evals = r["device_evals"] * a.steps
-
-    dom = max(kst, key=lambda n: kst[n]["ms"])
-    d = kst[dom]
-    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
-    traffic = load_traffic(dom)
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic, "kernel": dom,
-            "algorithmic_bytes_per_launch": d["bytes"] / d["launches"],
-            "mean_launch_us": d["ms"] * 1e3 / d["launches"], "launches": d["launches"],
-            "device_ms_share": d["ms"] / sum(x["ms"] for x in kst.values())}
+    evals = r["device_evals"]
     it = st["iterations"]
     out = {
-        "metric": METRIC, "value": conv / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": total * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": rate, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
         "data": "synthetic: seeded 64x64 R-diode mesh, per-instance Gaussian sigma 5% on every R and IS",
         "config": {"workload": f"rdmesh{a.side} Monte-Carlo DC operating point, plain FP64 batched GPU path",
                    "instances": B, "nodes": T.info["n"], "resistors": len(nl.res_ohms),
                    "diodes": len(nl.dio_is), "current_sources": len(nl.isrc_amps),
                    "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
-                   "gmin_steps": 19, "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": sweeps[-1]},
-        "nr_iters_per_solve": {"plain": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
-                                         "p99": float(np.percentile(it, 99))}, "seeded": None},
-        "device_evals_per_s": evals / total,
-        "roofline": roof,
-        "kernels": {n: {"launches": x["launches"], "ms": round(x["ms"], 3),
-                        "GBps": round(x["bytes"] / max(x["ms"], 1e-9) / 1e6, 1)} for n, x in
-                    sorted(kst.items(), key=lambda kv: -kv[1]["ms"])},
+                   "gmin_steps": 19, "symbolic_ms": symbolic_ms,
+                   "batched_sweeps_per_step": r["batched_iterations"]},
+        "nr_iters_per_solve": {"plain": iter_stats(it), "plain_fp32_refine": None, "seeded": None,
+                               "seeded_fp32_refine": None},
+        "device_evals_per_s": evals * 1e3 / ms,
+        "roofline": roofline(kst),
+        "kernels": kernel_table(kst),
         "cpu_baseline": None,
     }
+    # the FP32 path on the same instances, its error against the FP64 results
+    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)                # warm-up
+    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE)
+    V32 = bt.voltages()
+    out["nr_iters_per_solve"]["plain_fp32_refine"] = iter_stats(st32["iterations"])
+    out["fp32_refine"] = {"value": rate32, "unit": "solves/s", "ms_per_step": ms32,
+                          "speedup_vs_plain": rate32 / rate if rate else None,
+                          "max_err_over_tol_vs_plain_fp64": err_over_tol(V32, V),
+                          "max_kcl": float(st32["kcl"].max()),
+                          "batched_sweeps_per_step": r32["batched_iterations"],
+                          "roofline": roofline(kst32, traffic=False), "kernels": kernel_table(kst32)}
+    seeds = None
     if not a.no_seeder:
-        out["seeded"] = seeded_path(a, ctx, nl, T, bt, fields, B, it)
-        out["nr_iters_per_solve"]["seeded"] = out["seeded"].pop("iterations")
+        sd = seeded_path(a, ctx, nl, T, bt, fields, B, V)
+        seeds = sd.pop("_seed_sample")
+        out["seeded"] = sd
+        for key in ("fp64", "fp32_refine"):
+            sd[key]["speedup_vs_plain"] = sd[key]["value"] / rate if rate else None
+        out["nr_iters_per_solve"]["seeded"] = sd["fp64"].pop("iterations")
+        out["nr_iters_per_solve"]["seeded_fp32_refine"] = sd["fp32_refine"].pop("iterations")
+        out["target_speedup"] = {"seeded_fp32_refine_over_plain_fp64": sd["fp32_refine"]["speedup_vs_plain"],
+                                 "goal": 2.0}
     if not a.no_cpu:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         base, parity = cpu_baseline(nl, fields, B, V, it, a.cpu_seconds, threads)
         out["cpu_baseline"] = base
         out["parity_sample"] = parity
-        if "seeded" in out:
-            out["seeded"]["parity_sample"] = seeded_parity(nl, fields, out["seeded"])
+        if seeds is not None:
+            for key in ("fp64", "fp32_refine"):
+                out["seeded"][key]["parity_sample"] = seeded_parity(nl, fields, seeds, out["seeded"][key])
         log(f"cpu baseline {base['value']:.1f} solves/s on {threads} threads; parity {parity}")
     if "seeded" in out:
-        sd = out["seeded"]
-        sd["speedup_vs_plain"] = sd["value"] / out["value"] if out["value"] else None
-        for k in [k for k in sd if k.startswith("_")]:
-            del sd[k]
+        for key in ("fp64", "fp32_refine"):
+            for k in [k for k in out["seeded"][key] if k.startswith("_")]:
+                del out["seeded"][key][k]
     print(json.dumps(out), flush=True)
     return 0
 

@@ -195,6 +195,13 @@ NR_API int nr_batch_set_params(nr_batch* batch, int32_t field,
 NR_API int nr_batch_set_seed(nr_batch* batch, const double* v0, int64_t count);
 NR_API int nr_batch_clear_seed(nr_batch* batch);
 
+/* NR_PREC_FP64: the reference's FP64 arithmetic throughout.
+ * NR_PREC_FP32_REFINE: device evaluation (the Jacobian entries), assembly and
+ * the LU factorisation and solves in FP32; each Newton step solves
+ * J dv = b - A v, with the Newton residual b - A v formed in FP64 from the
+ * KCL audit vector (the refinement), and updates v += dv in FP64; pnjlim,
+ * damping, the KCL audit and the convergence test stay FP64.  Same control
+ * flow, iteration accounting and statuses as the FP64 path. */
 enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
 
 typedef struct nr_solve_opts {
@@ -250,6 +257,7 @@ typedef struct nr_transient_opts {
     int32_t n_probe;         /* nodes recorded at every step                 */
     const int32_t* probe;    /* 1-based node ids, n_probe of them            */
     int32_t record_timing;   /* as in nr_solve_opts                          */
+    int32_t precision;       /* NR_PREC_*, as in nr_solve_opts               */
 } nr_transient_opts;
 NR_API void nr_transient_opts_default(nr_transient_opts* opts);
 

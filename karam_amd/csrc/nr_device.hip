@@ -144,6 +144,11 @@ struct DevBatch {
     double *cap, *cap_vprev, *vsrc_t;
     const double* vnow;
     double h;
+    // FP32 path (p32 = 1): the FP64 KCL vector of the current iterate
+    // (instance-major); its interleaved copy lives in rhslin, which the FP32
+    // path does not otherwise use
+    double* res;
+    int p32;
     InstState* st;
     int* act;
     int* nact;
@@ -312,10 +317,22 @@ __device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const dou
     return s;
 }
 
-__device__ double block_kcl(const DevTopo& T, const DevBatch& P, int b, const double* v, double* red) {
+// max |KCL| over the nodes of instance b; `out` (optional) receives the
+// per-node values
+__device__ double block_kcl(const DevTopo& T, const DevBatch& P, int b, const double* v, double* red,
+                            double* out = nullptr) {
     double mx = 0.0;
-    for (int i = threadIdx.x; i < T.n; i += blockDim.x) mx = ref_max(mx, fabs(kcl_node(T, P, b, v, i)));
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) {
+        const double r = kcl_node(T, P, b, v, i);
+        if (out) out[i] = r;
+        mx = ref_max(mx, fabs(r));
+    }
     return block_max(mx, red);
+}
+
+// FP32 path: the interleaved KCL vector at v (one block per instance)
+__device__ void residual_to_interleaved(const DevTopo& T, const DevBatch& P, int b, const double* v) {
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = kcl_node(T, P, b, v, i);
 }
 
 // linear rhs of node i in emission order: Norton V sources (at the values in
@@ -397,6 +414,7 @@ __global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P) {
     double* v = P.v + (size_t)b * T.n;
     seed_initial_guess(T, P, b, v);
     for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.vI[IX(i, b, P.Bp)] = v[i];
+    if (P.p32) residual_to_interleaved(T, P, b, v);
     if (threadIdx.x == 0) {
         st.step = 0; st.it = 0;
         st.status = NR_STATUS_NOT_CONVERGED; st.done = 0; st.singular = 0; st.seeded = 0;
@@ -455,7 +473,8 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         for (int i = threadIdx.x; i < T.nC; i += blockDim.x)
             P.cap_vprev[(size_t)b * T.nC + i] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
     }
-    if (mode == 0)
+    if (mode == 0 && P.p32) residual_to_interleaved(T, P, b, v);
+    else if (mode == 0)
         for (int i = threadIdx.x; i < n; i += blockDim.x) P.rhslin[IX(i, b, Bp)] = rhs_lin_node(T, P, b, i);
     if (threadIdx.x == 0) {
         InstState s;
@@ -478,7 +497,9 @@ __global__ __launch_bounds__(256) void k_step_begin(DevTopo T, DevBatch P, doubl
         P.vsrc_t[(size_t)b * T.nV + i] =
             T.vsrc_is_pulse[i] ? pulse_value(T.vsrc_pulse + 7 * i, t_next) : P.vsrc[(size_t)b * T.nV + i];
     __syncthreads();
-    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
+    if (P.p32) residual_to_interleaved(T, P, b, P.v + (size_t)b * T.n);
+    else
+        for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
 }
 
 // Transient step close (one block per instance): an instance whose step
@@ -586,6 +607,106 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
     }
 }
 
+// FP32 path: the Jacobian entries of the generic device kernels
+// (tensors/physics_tensors.cpp:19-104) in single precision.  Junction
+// voltages are differenced in FP64 and rounded once, parameters are rounded to
+// float; only the conductances are produced (the rhs is the FP64 KCL vector).
+__device__ __forceinline__ float diode_g32(double vd64, double Is, double N, double Vt) {
+#pragma clang fp contract(off)
+    const float vd = (float)vd64, n_vt = (float)N * (float)Vt;
+    const float arg = vd / n_vt;
+    const float argc = (30.0f < arg) ? 30.0f : arg;
+    const float gb = ((float)Is / n_vt) * expf(argc);
+    return (1e-12f < gb) ? gb : 1e-12f;
+}
+
+__device__ __forceinline__ void mos_g32(double vgs64, double vds64, double W, double L, double Kp, double Vth,
+                                        double lam, int pmos, float& gm, float& gds) {
+#pragma clang fp contract(off)
+    const float sign = pmos ? -1.0f : 1.0f;
+    const float lvgs = sign * (float)vgs64, lvds = sign * (float)vds64, lvth = sign * (float)Vth;
+    const float K = 0.5f * (float)Kp * ((float)W / (float)L);
+    const float vov = lvgs - lvth;
+    if (vov <= 0.0f) {
+        gm = 0.0f; gds = 1e-12f;
+    } else if (lvds < vov) {
+        gm = 2.0f * K * lvds;
+        gds = 2.0f * K * (vov - lvds);
+    } else {
+        gm = 2.0f * K * vov * (1.0f + (float)lam * lvds);
+        gds = K * vov * vov * (float)lam;
+    }
+    if (gds < 1e-12f) gds = 1e-12f;
+}
+
+__global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+#pragma clang fp contract(off)
+    LANE_SETUP(4)
+    const int n = T.n;
+    auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
+    if (item < T.nD) {
+        const int i = item;
+        reinterpret_cast<float*>(P.dg)[IX(i, b, Bp)] =
+            diode_g32(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], T.d_vt[i]);
+    } else if (item < T.nD + T.nM) {
+        const int j = item - T.nD;
+        const double vs = V(T.m_s[j]);
+        float gm, gds;
+        mos_g32(V(T.m_g[j]) - vs, V(T.m_d[j]) - vs, T.m_w[j], T.m_l[j], P.kpI[IX(j, b, Bp)], P.vthI[IX(j, b, Bp)],
+                T.m_lam[j], T.m_pmos[j], gm, gds);
+        reinterpret_cast<float*>(P.mgds)[IX(j, b, Bp)] = gds;
+        reinterpret_cast<float*>(P.mgm)[IX(j, b, Bp)] = gm;
+    }
+}
+
+// FP32 path: A in float (the linear part, summed in FP64, rounded once; then
+// the FP32 device stamps and the gmin diagonal in emission order) and the
+// Newton rhs b - A v = KCL vector - gmin v, formed in FP64 and rounded
+__global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+                                                    const double* __restrict__ gmin_tab, int gmin_last) {
+#pragma clang fp contract(off)
+    LANE_SETUP(4)
+    float* pool = reinterpret_cast<float*>(P.pool);
+    const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
+    if (item < T.nnz_a) {
+        const float* dg = reinterpret_cast<const float*>(P.dg);
+        const float* mgds = reinterpret_cast<const float*>(P.mgds);
+        const float* mgm = reinterpret_cast<const float*>(P.mgm);
+        float a = (float)P.glin[IX(item, b, Bp)];
+        for (int k = T.nl_ptr[item]; k < T.nl_ptr[item + 1]; ++k) {
+            const int t = T.nl_term[k];
+            const int idx = term_idx(t);
+            float g;
+            switch (term_kind(t)) {
+            case karam::SK_DIODE: g = dg[IX(idx, b, Bp)]; break;
+            case karam::SK_MOS_GDS: g = mgds[IX(idx, b, Bp)]; break;
+            default: g = mgm[IX(idx, b, Bp)]; break;
+            }
+            a = term_neg(t) ? a + (-g) : a + g;
+        }
+        if (T.slot_diag[item]) a = a + (float)gmin;
+        const int d = T.slot_dst[item];
+        if (d & 1) reinterpret_cast<float*>(P.Al)[(size_t)b * T.nAl + (d >> 1)] = a;
+        else pool[IX(d >> 1, b, Bp)] = a;
+    }
+    if (item < T.n) {
+        const float r = (float)(P.rhslin[IX(item, b, Bp)] - gmin * P.vI[IX(item, b, Bp)]);
+        const int d = T.node_dst[item];
+        if (d & 1) reinterpret_cast<float*>(P.rhsl)[(size_t)b * T.n + (d >> 1)] = r;
+        else pool[IX(d >> 1, b, Bp)] = r;
+    }
+}
+
+// value-type helpers of the multifrontal kernels (V = double: the FP64 path;
+// V = float: the FP32 path, same operation order)
+__device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double vabs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
+// the factor buffers are sized for FP64; the FP32 path uses their first half
+template <typename V>
+__device__ __forceinline__ V* vp(double* p) { return reinterpret_cast<V*>(p); }
+
 // ---------------------------------------------------------------------------
 // kernels: small fronts (lane = instance, front in VGPRs)
 // ---------------------------------------------------------------------------
@@ -594,7 +715,7 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
 // gather table holds, for the A entries and then for each child, one pool
 // item per front position (the pool's zero row where nothing contributes),
 // so every load is unconditional and the sum order matches the oracle's.
-template <int R>
+template <int R, typename V>
 __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, int J, int b, bool on) {
 #pragma clang fp contract(off)
     constexpr int TB = R * R + R;
@@ -602,14 +723,14 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
     const int nch = T.sn_child_end[J] - T.sn_child_begin[J];
     const int* tb = T.tb + T.tb_off[J];
-    const double* pool = P.pool;
+    const V* pool = vp<V>(P.pool);
     const int bb = on ? b : 0;
-    double F[R * R];
-    double fr[R];
+    V F[R * R];
+    V fr[R];
 #pragma unroll
-    for (int p = 0; p < R * R; ++p) F[p] = 0.0 + pool[IX(tb[p], bb, Bp)];
+    for (int p = 0; p < R * R; ++p) F[p] = V(0) + pool[IX(tb[p], bb, Bp)];
 #pragma unroll
-    for (int i = 0; i < R; ++i) fr[i] = 0.0 + pool[IX(tb[R * R + i], bb, Bp)];
+    for (int i = 0; i < R; ++i) fr[i] = V(0) + pool[IX(tb[R * R + i], bb, Bp)];
     for (int k = 0; k < nch; ++k) {
         const int* t2 = tb + (k + 1) * TB;
 #pragma unroll
@@ -621,39 +742,39 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
 #pragma unroll
     for (int k = 0; k < R; ++k) {
         if (k < c) {
-            const double piv = F[k * R + k];
-            if (!(fabs(piv) >= 1e-25)) sing = 1;
+            const V piv = F[k * R + k];
+            if (!(vabs(piv) >= V(1e-25))) sing = 1;
 #pragma unroll
             for (int i = k + 1; i < R; ++i) F[i * R + k] = F[i * R + k] / piv;
 #pragma unroll
             for (int i = k + 1; i < R; ++i) {
-                const double l = F[i * R + k];
+                const V l = F[i * R + k];
 #pragma unroll
-                for (int j = k + 1; j < R; ++j) F[i * R + j] = fma(-l, F[k * R + j], F[i * R + j]);
-                fr[i] = fma(-l, fr[k], fr[i]);
+                for (int j = k + 1; j < R; ++j) F[i * R + j] = vfma(-l, F[k * R + j], F[i * R + j]);
+                fr[i] = vfma(-l, fr[k], fr[i]);
             }
         }
     }
     if (on) {
         const long long uo = T.sn_u_off[J], co = T.sn_cb_off[J], ro = T.sn_rcb_off[J];
         const bool root = T.sn_root[J] != 0;
-        double* cbl = P.CBL + (size_t)b * T.cbL + co;
-        double* rcbl = P.RCBL + (size_t)b * T.rcbL + ro;
+        V* cbl = vp<V>(P.CBL) + (size_t)b * T.cbL + co;
+        V* rcbl = vp<V>(P.RCBL) + (size_t)b * T.rcbL + ro;
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             if (i < c) {
-                P.ys[IX(f + i, b, Bp)] = fr[i];
+                vp<V>(P.ys)[IX(f + i, b, Bp)] = fr[i];
 #pragma unroll
                 for (int j = 0; j < R; ++j)
-                    if (j < r) P.Us[IX(uo + i * r + j, b, Bp)] = F[i * R + j];
+                    if (j < r) vp<V>(P.Us)[IX(uo + i * r + j, b, Bp)] = F[i * R + j];
             } else if (i < r) {
                 if (root) rcbl[i - c] = fr[i];
-                else P.pool[IX(ro + (i - c), b, Bp)] = fr[i];
+                else vp<V>(P.pool)[IX(ro + (i - c), b, Bp)] = fr[i];
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     if (j >= c && j < r) {
                         if (root) cbl[(i - c) * m + (j - c)] = F[i * R + j];
-                        else P.pool[IX(co + (i - c) * m + (j - c), b, Bp)] = F[i * R + j];
+                        else vp<V>(P.pool)[IX(co + (i - c) * m + (j - c), b, Bp)] = F[i * R + j];
                     }
                 }
             }
@@ -662,7 +783,7 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
     return sing;
 }
 
-template <int R>
+template <int R, typename V>
 __device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, int J, int b, bool on) {
 #pragma clang fp contract(off)
     const int Bp = P.Bp;
@@ -670,22 +791,22 @@ __device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, 
     const long long uo = T.sn_u_off[J];
     const int* rows = T.row_idx + T.sn_row_off[J];
     const int bb = on ? b : 0;
-    double t[R];
+    V t[R];
 #pragma unroll
-    for (int k = 0; k < R; ++k) t[k] = (k < c) ? P.ys[IX(f + k, bb, Bp)] : 0.0;
+    for (int k = 0; k < R; ++k) t[k] = (k < c) ? vp<V>(P.ys)[IX(f + k, bb, Bp)] : V(0);
     for (int a = 0; a < m; ++a) {
-        const double xa = P.xs[IX(rows[a], bb, Bp)];
+        const V xa = vp<V>(P.xs)[IX(rows[a], bb, Bp)];
 #pragma unroll
         for (int k = 0; k < R; ++k)
-            if (k < c) t[k] = fma(-P.Us[IX(uo + k * r + c + a, bb, Bp)], xa, t[k]);
+            if (k < c) t[k] = vfma(-vp<V>(P.Us)[IX(uo + k * r + c + a, bb, Bp)], xa, t[k]);
     }
 #pragma unroll
     for (int k = R - 1; k >= 0; --k) {
         if (k < c) {
-            const double xk = t[k] / P.Us[IX(uo + k * r + k, bb, Bp)];
-            if (on) P.xs[IX(f + k, b, Bp)] = xk;
+            const V xk = t[k] / vp<V>(P.Us)[IX(uo + k * r + k, bb, Bp)];
+            if (on) vp<V>(P.xs)[IX(f + k, b, Bp)] = xk;
 #pragma unroll
-            for (int i = 0; i < k; ++i) t[i] = fma(-P.Us[IX(uo + i * r + k, bb, Bp)], xk, t[i]);
+            for (int i = 0; i < k; ++i) t[i] = vfma(-vp<V>(P.Us)[IX(uo + i * r + k, bb, Bp)], xk, t[i]);
         }
     }
 }
@@ -693,7 +814,7 @@ __device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, 
 // one wavefront per (small subtree, 64 instances); 4 wavefronts per block.
 // Every front of the subtrees in one launch is padded to RMAX, so each
 // kernel carries a single front shape (register budget and code size).
-template <int RMAX>
+template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
                                                       const int* __restrict__ sub_ptr,
                                                       const int* __restrict__ sub_sn, int n_sub) {
@@ -706,11 +827,11 @@ __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, con
     const bool on = slot < nact;
     const int b = on ? act[slot] : 0;
     int sing = 0;
-    for (int q = sub_ptr[sub]; q < sub_ptr[sub + 1]; ++q) sing |= front_small<RMAX>(T, P, sub_sn[q], b, on);
+    for (int q = sub_ptr[sub]; q < sub_ptr[sub + 1]; ++q) sing |= front_small<RMAX, V>(T, P, sub_sn[q], b, on);
     if (on && sing) P.st[b].singular = 1;
 }
 
-template <int RMAX>
+template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                          int nact, const int* __restrict__ sub_ptr,
                                                          const int* __restrict__ sub_sn, int n_sub) {
@@ -722,7 +843,7 @@ __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, 
     const int slot = g * 64 + lane;
     const bool on = slot < nact;
     const int b = on ? act[slot] : 0;
-    for (int q = sub_ptr[sub + 1] - 1; q >= sub_ptr[sub]; --q) back_small<RMAX>(T, P, sub_sn[q], b, on);
+    for (int q = sub_ptr[sub + 1] - 1; q >= sub_ptr[sub]; --q) back_small<RMAX, V>(T, P, sub_sn[q], b, on);
 }
 
 // ---------------------------------------------------------------------------
@@ -755,11 +876,12 @@ __device__ __forceinline__ void team_sync() {
 //       4x4 register tiles and streamed to HBM.  Without FULL its entries are
 //       gathered straight from the children's update blocks (every A entry of
 //       a front lies in a pivot row or column, so only children contribute).
-template <int TEAM, int NT, bool FULL>
+template <int TEAM, int NT, bool FULL, typename V>
 __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
                                                      const int* __restrict__ sn_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
-    extern __shared__ double smem[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
     const int tib = threadIdx.x / TEAM;
     const int lane = threadIdx.x % TEAM;
@@ -774,26 +896,26 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     if (slot >= nact) return;
     const int b = act[slot];
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    double* Pr = smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
+    V* Pr = smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
     // front-local (i, j) with i < c or j < c; rows c.. of the pivot columns
     // have stride r in the full front, c in the panel
     const int ls = FULL ? r : c;
-    double* Lc = FULL ? Pr + c * r : Pr + c * r;           // row i of L21 at Lc + i * ls
-    double* fr = FULL ? Pr + r * r : Pr + c * r + m * c;   // rhs, r
+    V* Lc = FULL ? Pr + c * r : Pr + c * r;           // row i of L21 at Lc + i * ls
+    V* fr = FULL ? Pr + r * r : Pr + c * r + m * c;   // rhs, r
     auto pidx = [&](int i, int j) { return i < c ? i * r + j : c * r + (i - c) * ls + j; };
     const int np = FULL ? r * r + r : c * r + m * c + r;
-    for (int e = lane; e < np; e += TEAM) Pr[e] = 0.0;
+    for (int e = lane; e < np; e += TEAM) Pr[e] = V(0);
     team_sync<TEAM>();
     // (A) assembly: A entries, the pivot rows' rhs, then each child in order
-    const double* Ab = P.Al + (size_t)b * T.nAl;
+    const V* Ab = vp<V>(P.Al) + (size_t)b * T.nAl;
     {
         const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
         for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
-            double vals[8];
+            V vals[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * TEAM;
-                vals[u] = (e < a1) ? Ab[T.fa_slot[e]] : 0.0;
+                vals[u] = (e < a1) ? Ab[T.fa_slot[e]] : V(0);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -802,7 +924,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
             }
         }
     }
-    const double* rb = P.rhsl + (size_t)b * T.n;
+    const V* rb = vp<V>(P.rhsl) + (size_t)b * T.n;
     for (int k = lane; k < c; k += TEAM) fr[k] += rb[T.perm[f + k]];
     team_sync<TEAM>();
     const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
@@ -810,13 +932,13 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
         const int C = T.children[ch];
         const int mc = T.sn_nrows[C], pc = FULL ? mc : T.sn_pc[C];
         const int* rel = T.relmap + T.sn_row_off[C];
-        const double* cb = P.CBL + (size_t)b * T.cbL + T.sn_cb_off[C];
-        const double* rcb = P.RCBL + (size_t)b * T.rcbL + T.sn_rcb_off[C];
+        const V* cb = vp<V>(P.CBL) + (size_t)b * T.cbL + T.sn_cb_off[C];
+        const V* rcb = vp<V>(P.RCBL) + (size_t)b * T.rcbL + T.sn_rcb_off[C];
         // FULL: every entry; else rows a < pc (all columns) and columns q < pc
         // of the remaining rows (the child's entries that land in the panel)
         const int n1 = pc * mc, nn = n1 + (mc - pc) * pc;
         for (int e0 = lane; e0 < nn; e0 += 8 * TEAM) {
-            double vals[8];
+            V vals[8];
             int pos[8];
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -824,7 +946,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 int a = 0, q = 0;
                 if (e < n1) { a = e / mc; q = e - a * mc; }
                 else if (e < nn) { const int e2 = e - n1; a = pc + e2 / pc; q = e2 - (a - pc) * pc; }
-                vals[u] = (e < nn) ? cb[a * mc + q] : 0.0;
+                vals[u] = (e < nn) ? cb[a * mc + q] : V(0);
                 pos[u] = (e < nn) ? (FULL ? rel[a] * r + rel[q] : pidx(rel[a], rel[q])) : 0;
             }
 #pragma unroll
@@ -843,26 +965,26 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const int jl = lane % SW, st = lane / SW;
     int sing = 0;
     for (int k = 0; k < c; ++k) {
-        const double piv = Pr[k * r + k];
-        if (!(fabs(piv) >= 1e-25)) sing = 1;
+        const V piv = Pr[k * r + k];
+        if (!(vabs(piv) >= V(1e-25))) sing = 1;
         for (int i = k + 1 + lane; i < c; i += TEAM) Pr[i * r + k] = Pr[i * r + k] / piv;
         team_sync<TEAM>();
         for (int j = k + 1 + jl; j < c; j += SW) {
-            const double u = Pr[k * r + j];
+            const V u = Pr[k * r + j];
             int i = k + 1 + st;
             for (; i + 3 * NS < c; i += 4 * NS) {
-                const double l0 = Pr[i * r + k], l1 = Pr[(i + NS) * r + k];
-                const double l2 = Pr[(i + 2 * NS) * r + k], l3 = Pr[(i + 3 * NS) * r + k];
-                const double a0 = Pr[i * r + j], a1 = Pr[(i + NS) * r + j];
-                const double a2 = Pr[(i + 2 * NS) * r + j], a3 = Pr[(i + 3 * NS) * r + j];
-                Pr[i * r + j] = fma(-l0, u, a0);
-                Pr[(i + NS) * r + j] = fma(-l1, u, a1);
-                Pr[(i + 2 * NS) * r + j] = fma(-l2, u, a2);
-                Pr[(i + 3 * NS) * r + j] = fma(-l3, u, a3);
+                const V l0 = Pr[i * r + k], l1 = Pr[(i + NS) * r + k];
+                const V l2 = Pr[(i + 2 * NS) * r + k], l3 = Pr[(i + 3 * NS) * r + k];
+                const V a0 = Pr[i * r + j], a1 = Pr[(i + NS) * r + j];
+                const V a2 = Pr[(i + 2 * NS) * r + j], a3 = Pr[(i + 3 * NS) * r + j];
+                Pr[i * r + j] = vfma(-l0, u, a0);
+                Pr[(i + NS) * r + j] = vfma(-l1, u, a1);
+                Pr[(i + 2 * NS) * r + j] = vfma(-l2, u, a2);
+                Pr[(i + 3 * NS) * r + j] = vfma(-l3, u, a3);
             }
-            for (; i < c; i += NS) Pr[i * r + j] = fma(-Pr[i * r + k], u, Pr[i * r + j]);
+            for (; i < c; i += NS) Pr[i * r + j] = vfma(-Pr[i * r + k], u, Pr[i * r + j]);
         }
-        for (int i = k + 1 + lane; i < c; i += TEAM) fr[i] = fma(-Pr[i * r + k], fr[k], fr[i]);
+        for (int i = k + 1 + lane; i < c; i += TEAM) fr[i] = vfma(-Pr[i * r + k], fr[k], fr[i]);
         team_sync<TEAM>();
     }
     // (C) U12 columns and L21 rows, each by one lane
@@ -870,15 +992,15 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
         if (t < m) {
             const int j = c + t;
             for (int k = 1; k < c; ++k) {
-                double x = Pr[k * r + j];
-                for (int p = 0; p < k; ++p) x = fma(-Pr[k * r + p], Pr[p * r + j], x);
+                V x = Pr[k * r + j];
+                for (int p = 0; p < k; ++p) x = vfma(-Pr[k * r + p], Pr[p * r + j], x);
                 Pr[k * r + j] = x;
             }
         } else {
-            double* Li = Lc + (t - m) * ls;
+            V* Li = Lc + (t - m) * ls;
             for (int k = 0; k < c; ++k) {
-                double x = Li[k];
-                for (int p = 0; p < k; ++p) x = fma(-Li[p], Pr[p * r + k], x);
+                V x = Li[k];
+                for (int p = 0; p < k; ++p) x = vfma(-Li[p], Pr[p * r + k], x);
                 Li[k] = x / Pr[k * r + k];
             }
         }
@@ -886,40 +1008,40 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     team_sync<TEAM>();
     // (D) trailing block in 4x4 register tiles, streamed to the update block
     if (m > 0) {
-        double* cbJ = P.CBL + (size_t)b * T.cbL + T.sn_cb_off[J];
-        const double* cbase = P.CBL + (size_t)b * T.cbL;
+        V* cbJ = vp<V>(P.CBL) + (size_t)b * T.cbL + T.sn_cb_off[J];
+        const V* cbase = vp<V>(P.CBL) + (size_t)b * T.cbL;
         const int mt = (m + 3) >> 2;
         for (int tt = lane; tt < mt * mt; tt += TEAM) {
             const int i0 = (tt / mt) * 4, j0 = (tt % mt) * 4;
-            double acc[4][4];
+            V acc[4][4];
             if constexpr (FULL) {
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        acc[a][q] = (i0 + a < m && j0 + q < m) ? Pr[(c + i0 + a) * r + c + j0 + q] : 0.0;
+                        acc[a][q] = (i0 + a < m && j0 + q < m) ? Pr[(c + i0 + a) * r + c + j0 + q] : V(0);
             } else {
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[a][q] = 0.0;
+                    for (int q = 0; q < 4; ++q) acc[a][q] = V(0);
                 for (int ch = ch0; ch < ch1; ++ch) {
                     const int C = T.children[ch];
                     const int mc = T.sn_nrows[C];
                     const int* inv = T.inv + T.inv_off[C] + c;
-                    const double* cb = cbase + T.sn_cb_off[C];
+                    const V* cb = cbase + T.sn_cb_off[C];
                     int ra[4], qa[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
                         ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
                         qa[a] = (j0 + a < m) ? inv[j0 + a] : -1;
                     }
-                    double x[4][4];
+                    V x[4][4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a)
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
-                            x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : 0.0;
+                            x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : V(0);
 #pragma unroll
                     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -928,15 +1050,15 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 }
             }
             for (int k = 0; k < c; ++k) {
-                double l[4], u[4];
+                V l[4], u[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) l[a] = (i0 + a < m) ? Lc[(i0 + a) * ls + k] : 0.0;
+                for (int a = 0; a < 4; ++a) l[a] = (i0 + a < m) ? Lc[(i0 + a) * ls + k] : V(0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) u[q] = (j0 + q < m) ? Pr[k * r + c + j0 + q] : 0.0;
+                for (int q = 0; q < 4; ++q) u[q] = (j0 + q < m) ? Pr[k * r + c + j0 + q] : V(0);
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[a][q] = fma(-l[a], u[q], acc[a][q]);
+                    for (int q = 0; q < 4; ++q) acc[a][q] = vfma(-l[a], u[q], acc[a][q]);
             }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
@@ -944,26 +1066,27 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 for (int q = 0; q < 4; ++q)
                     if (i0 + a < m && j0 + q < m) cbJ[(i0 + a) * m + j0 + q] = acc[a][q];
         }
-        double* rcb = P.RCBL + (size_t)b * T.rcbL + T.sn_rcb_off[J];
+        V* rcb = vp<V>(P.RCBL) + (size_t)b * T.rcbL + T.sn_rcb_off[J];
         for (int i = lane; i < m; i += TEAM) {
-            double acc = fr[c + i];
-            for (int k = 0; k < c; ++k) acc = fma(-Lc[i * ls + k], fr[k], acc);
+            V acc = fr[c + i];
+            for (int k = 0; k < c; ++k) acc = vfma(-Lc[i * ls + k], fr[k], acc);
             rcb[i] = acc;
         }
     }
-    double* U = P.UL + (size_t)b * T.uL + T.sn_u_off[J];
+    V* U = vp<V>(P.UL) + (size_t)b * T.uL + T.sn_u_off[J];
     for (int e = lane; e < c * r; e += TEAM) U[e] = Pr[e];
-    double* yb = P.yl + (size_t)b * T.n;
+    V* yb = vp<V>(P.yl) + (size_t)b * T.n;
     for (int k = lane; k < c; k += TEAM) yb[f + k] = fr[k];
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
-template <int TEAM, int NT>
+template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                         int nact, const int* __restrict__ sn_list, int n_sn,
                                                         int rmax) {
 #pragma clang fp contract(off)
-    extern __shared__ double smem[];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
     const int tib = threadIdx.x / TEAM;
     const int lane = threadIdx.x % TEAM;
@@ -979,26 +1102,26 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     const int b = act[slot];
     const int Bp = P.Bp;
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    double* xl = smem + (size_t)tib * (size_t)rmax;
-    const double* U = P.UL + (size_t)b * T.uL + T.sn_u_off[J];
+    V* xl = smem + (size_t)tib * (size_t)rmax;
+    const V* U = vp<V>(P.UL) + (size_t)b * T.uL + T.sn_u_off[J];
     const int* rows = T.row_idx + T.sn_row_off[J];
-    double* xb = P.xl + (size_t)b * T.n;
-    const double* yb = P.yl + (size_t)b * T.n;
+    V* xb = vp<V>(P.xl) + (size_t)b * T.n;
+    const V* yb = vp<V>(P.yl) + (size_t)b * T.n;
     for (int a = lane; a < m; a += TEAM) xl[c + a] = xb[rows[a]];
     team_sync<TEAM>();
     for (int k = lane; k < c; k += TEAM) {
-        double s = yb[f + k];
-        const double* Uk = U + (size_t)k * r + c;
-        for (int a = 0; a < m; ++a) s = fma(-Uk[a], xl[c + a], s);
+        V s = yb[f + k];
+        const V* Uk = U + (size_t)k * r + c;
+        for (int a = 0; a < m; ++a) s = vfma(-Uk[a], xl[c + a], s);
         xl[k] = s;
     }
     team_sync<TEAM>();
     for (int k = c - 1; k >= 0; --k) {
-        const double xk = xl[k] / U[(size_t)k * r + k];
-        for (int i = lane; i < k; i += TEAM) xl[i] = fma(-U[(size_t)i * r + k], xk, xl[i]);
+        const V xk = xl[k] / U[(size_t)k * r + k];
+        for (int i = lane; i < k; i += TEAM) xl[i] = vfma(-U[(size_t)i * r + k], xk, xl[i]);
         if (lane == 0) {
             xb[f + k] = xk;
-            P.xs[IX(f + k, b, Bp)] = xk;
+            vp<V>(P.xs)[IX(f + k, b, Bp)] = xk;
         }
         team_sync<TEAM>();
     }
@@ -1007,7 +1130,8 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
 // ---------------------------------------------------------------------------
 // kernels: layout bridges
 // ---------------------------------------------------------------------------
-// xs (interleaved, permuted rows) -> xm (instance-major, permuted rows)
+// xs (interleaved, permuted rows) -> xm (instance-major, permuted rows, FP64)
+template <typename V>
 __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1016,8 +1140,9 @@ __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const in
     const int n = T.n, Bp = P.Bp;
     if (slot < nact) {
         const int b = act[slot];
+        const V* xs = vp<V>(P.xs);
         for (int rr = w; rr < 64; rr += 4)
-            if (row0 + rr < n) tile[rr][lane] = P.xs[IX(row0 + rr, b, Bp)];
+            if (row0 + rr < n) tile[rr][lane] = (double)xs[IX(row0 + rr, b, Bp)];
     }
     __syncthreads();
     for (int ii = w; ii < 64; ii += 4) {
@@ -1028,24 +1153,30 @@ __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const in
     }
 }
 
-// v (instance-major) -> vI (interleaved)
+// v (instance-major) -> vI (interleaved); on the FP32 path also the KCL
+// vector res -> rhslin
 __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64, g = blockIdx.y;
     const int n = T.n, Bp = P.Bp;
-    for (int ii = w; ii < 64; ii += 4) {
-        const int s2 = g * 64 + ii;
-        if (s2 >= nact) break;
-        const int b = act[s2];
-        if (row0 + lane < n) tile[lane][ii] = P.v[(size_t)b * n + row0 + lane];
-    }
-    __syncthreads();
     const int slot = g * 64 + lane;
-    if (slot < nact) {
-        const int b = act[slot];
-        for (int rr = w; rr < 64; rr += 4)
-            if (row0 + rr < n) P.vI[IX(row0 + rr, b, Bp)] = tile[rr][lane];
+    for (int pass = 0; pass < (P.p32 ? 2 : 1); ++pass) {
+        const double* src = pass ? P.res : P.v;
+        double* dst = pass ? P.rhslin : P.vI;
+        if (pass) __syncthreads();
+        for (int ii = w; ii < 64; ii += 4) {
+            const int s2 = g * 64 + ii;
+            if (s2 >= nact) break;
+            const int b = act[s2];
+            if (row0 + lane < n) tile[lane][ii] = src[(size_t)b * n + row0 + lane];
+        }
+        __syncthreads();
+        if (slot < nact) {
+            const int b = act[slot];
+            for (int rr = w; rr < 64; rr += 4)
+                if (row0 + rr < n) dst[IX(row0 + rr, b, Bp)] = tile[rr][lane];
+        }
     }
 }
 
@@ -1055,7 +1186,9 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
 // Per-instance NR control after a linear solve (one block per instance).
 // tran = 1: a transient step (no limiting, no damping, KCL_TOL = 1e-12, a
 // converged instance waits for the next step; solvers/circuitsim_solver.cpp:
-// 600-635 restated)
+// 600-635 restated).  On the FP32 path the solve returned the correction,
+// so the new iterate is v + x (FP64), and every KCL audit also stores its
+// per-node vector (the next Newton rhs).
 __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
                                                 int nsteps, int tran) {
 #pragma clang fp contract(off)
@@ -1080,8 +1213,9 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     double* v = P.v + (size_t)b * n;
     double* vn = P.vn + (size_t)b * n;
     const double* xm = P.xm + (size_t)b * n;
+    double* res = P.p32 ? P.res + (size_t)b * n : nullptr;
     for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        double val = xm[T.iperm[i]];
+        double val = P.p32 ? v[i] + xm[T.iperm[i]] : xm[T.iperm[i]];
         if (isnan(val) || isinf(val)) val = 0.0;   // sanitize_vector
         vn[i] = val;
     }
@@ -1107,12 +1241,12 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         }
         __syncthreads();
     }
-    double kcl = block_kcl(T, P, b, vn, red);
+    double kcl = block_kcl(T, P, b, vn, red, res);
     if (!tran && st.it > 0 && kcl >= st.prev_kcl) {
         for (int pass = 0; pass < 2; ++pass) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
             __syncthreads();
-            kcl = block_kcl(T, P, b, vn, red);
+            kcl = block_kcl(T, P, b, vn, red, res);
             if (!(kcl >= st.prev_kcl)) break;
         }
     }

@@ -230,6 +230,41 @@ void orc_eval_mosfets(int32_t count, const int32_t* d, const int32_t* g,
 }
 
 /* ------------------------------------------------------------------------ */
+/* FP32 path (device evaluation and LU in single precision, FP64 defect    */
+/* correction): the Jacobian entries from the generic-kernel equations      */
+/* (tensors/physics_tensors.cpp:19-104) evaluated in float.  Junction       */
+/* voltages are differenced in FP64 and rounded once; parameters are        */
+/* rounded to float; the currents are not needed (the rhs is the FP64 KCL). */
+/* ------------------------------------------------------------------------ */
+static float diode_g_f32(double vd64, double Is, double N, double Vt) {
+    const float vd = (float)vd64, n_vt = (float)N * (float)Vt;
+    const float arg = vd / n_vt;
+    const float argc = (30.0f < arg) ? 30.0f : arg;
+    const float gb = ((float)Is / n_vt) * expf(argc);
+    return (1e-12f < gb) ? gb : 1e-12f;
+}
+
+static void mos_g_f32(double vgs64, double vds64, double W, double L, double Kp, double Vth, double lam,
+                      int pmos, float* o_gm, float* o_gds) {
+    const float sign = pmos ? -1.0f : 1.0f;
+    const float lvgs = sign * (float)vgs64, lvds = sign * (float)vds64, lvth = sign * (float)Vth;
+    const float K = 0.5f * (float)Kp * ((float)W / (float)L);
+    const float vov = lvgs - lvth;
+    float gm, gds;
+    if (vov <= 0.0f) {
+        gm = 0.0f; gds = 1e-12f;
+    } else if (lvds < vov) {
+        gm = 2.0f * K * lvds;
+        gds = 2.0f * K * (vov - lvds);
+    } else {
+        gm = 2.0f * K * vov * (1.0f + (float)lam * lvds);
+        gds = K * vov * vov * (float)lam;
+    }
+    if (gds < 1e-12f) gds = 1e-12f;
+    *o_gm = gm; *o_gds = gds;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Topology: stamps, pattern, Min-Fill, symbolic                            */
 /* ------------------------------------------------------------------------ */
 struct orc_topo {
@@ -272,6 +307,8 @@ typedef struct {
     ivec rows, cols;
     double* vals; int32_t nv;      /* optional values */
     double* rhs;
+    int f32;                       /* FP32 path: device values from the FP32 evaluation */
+    int32_t n_lin;                 /* stamps of the linear elements (R, V, C), emitted first */
 } emitter;
 static inline void emit(emitter* e, int32_t r, int32_t c, double val) {
     if (e->vals) { e->vals[e->nv++] = val; }
@@ -343,11 +380,14 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
             if (b != 0) rhs[b - 1] -= ih;
         }
     }
+    e->n_lin = e->vals ? e->nv : e->rows.n;
     /* (6) device evaluation :2535-2537, (7) diodes :2543-2576 */
     for (int32_t i = 0; i < s->nD; ++i) {
         int32_t a = s->da[i], c = s->dc[i];
         double g = 0.0, ieq = 0.0;
-        if (e->vals) {
+        if (e->vals && e->f32) {
+            g = diode_g_f32(volt(v, n, a) - volt(v, n, c), s->dIs[i], s->dN[i], s->dVt[i]);
+        } else if (e->vals) {
             double vd, id, gd;
             orc_eval_diodes(1, &a, &c, &s->dIs[i], &s->dN[i], &s->dVt[i], v, n, &vd, &id, &gd);
             g = gd; ieq = id - g * vd;
@@ -366,7 +406,13 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     for (int32_t i = 0; i < s->nM; ++i) {
         int32_t d = s->md[i], g = s->mg[i], so = s->ms[i], b = s->mb[i];
         double gm = 0, gmb = 0, gds = 0, ieq = 0;
-        if (e->vals) {
+        if (e->vals && e->f32) {
+            const double vs_ = volt(v, n, so);
+            float gm32, gds32;
+            mos_g_f32(volt(v, n, g) - vs_, volt(v, n, d) - vs_, s->mW[i], s->mL[i], s->mKp[i], s->mVth[i],
+                      s->mLam[i], s->mP[i], &gm32, &gds32);
+            gm = gm32; gds = gds32;
+        } else if (e->vals) {
             double vgs, vds, ids;
             mos_one(volt(v, n, g), volt(v, n, d), volt(v, n, so), s->mW[i], s->mL[i],
                     s->mKp[i], s->mVth[i], s->mLam[i], s->mP[i], &vgs, &vds, &ids, &gm, &gds);
@@ -400,7 +446,7 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
         }
     }
     /* gmin diagonal :2857-2860 */
-    for (int32_t i = 0; i < n; ++i) emit(e, i, i, gmin);
+    for (int32_t i = 0; i < n; ++i) emit(e, i, i, e->f32 ? (double)(float)gmin : gmin);
 }
 
 /* Sparse adjacency with O(1) membership (bitset) for graph elimination. */
@@ -815,6 +861,24 @@ int orc_assemble(const orc_topo* t, const double* vals, double* a_vals) {
     return NR_OK;
 }
 
+/* FP32 path: the Jacobian in float.  The linear elements' stamps (emitted
+ * first) are summed per slot in FP64 and rounded once; the FP32 device
+ * stamps and the gmin diagonal are then added in float, in emission order. */
+static int assemble_f32(const nr_netlist* nl, const orc_soa* s, const orc_topo* t, const double* v, double gmin,
+                        const tran_ctx* tc, double* vals, double* lin, float* a32) {
+    emitter e;
+    memset(&e, 0, sizeof(e));
+    e.vals = vals;
+    e.f32 = 1;
+    stamp_all(nl, s, v, gmin, &e, tc);
+    if (e.nv != t->n_stamps) return NR_ERR_TOPOLOGY;
+    for (int32_t q = 0; q < t->nnz_a; ++q) lin[q] = 0.0;
+    for (int32_t k = 0; k < e.n_lin; ++k) lin[t->stamp_slot[k]] += vals[k];
+    for (int32_t q = 0; q < t->nnz_a; ++q) a32[q] = (float)lin[q];
+    for (int32_t k = e.n_lin; k < t->n_stamps; ++k) a32[t->stamp_slot[k]] = a32[t->stamp_slot[k]] + (float)vals[k];
+    return NR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* a12: linear solves                                                       */
 /* ------------------------------------------------------------------------ */
@@ -1015,6 +1079,118 @@ int orc_lu_multifrontal(const orc_topo* t, const double* a_vals, const double* r
     return rc;
 }
 
+/* The same multifrontal LU in single precision (float storage, float FMAs,
+ * the same operation order), for the FP32 path. */
+int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float* rhs, float* x) {
+    const int32_t n = t->n, ns = t->ns;
+    int64_t* u_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* cb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* rcb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t ut = 0, ct = 0, rt = 0;
+    int32_t rmax = 1;
+    for (int32_t J = 0; J < ns; ++J) {
+        int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m;
+        u_off[J] = ut; ut += (int64_t)c * r;
+        cb_off[J] = ct; ct += (int64_t)m * m;
+        rcb_off[J] = rt; rt += m;
+        if (r > rmax) rmax = r;
+    }
+    float* U = (float*)xmalloc(4 * (size_t)(ut + 1));
+    float* CB = (float*)xmalloc(4 * (size_t)(ct + 1));
+    float* RCB = (float*)xmalloc(4 * (size_t)(rt + 1));
+    float* y = (float*)xmalloc(4 * (size_t)n);
+    float* xp = (float*)xmalloc(4 * (size_t)n);
+    float* F = (float*)xmalloc(4 * (size_t)rmax * (size_t)rmax);
+    float* fr = (float*)xmalloc(4 * (size_t)rmax);
+    int rc = NR_OK;
+    for (int32_t J = 0; J < ns; ++J) {
+        const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+        for (int32_t e = 0; e < r * r; ++e) F[e] = 0.0f;
+        for (int32_t e = 0; e < r; ++e) fr[e] = 0.0f;
+        for (int32_t q = t->sn_a_ptr[J]; q < t->sn_a_ptr[J + 1]; ++q)
+            F[t->sn_a_li[q] * r + t->sn_a_lj[q]] += a_vals[t->sn_a_slot[q]];
+        for (int32_t k = 0; k < c; ++k) fr[k] += rhs[t->perm[f + k]];
+        for (int32_t q = t->sn_child_ptr[J]; q < t->sn_child_ptr[J + 1]; ++q) {
+            const int32_t C = t->sn_child[q], mc = t->sn_nrows[C];
+            const int32_t* Rc = t->sn_rows + t->sn_row_off[C];
+            int32_t* rel = (int32_t*)xmalloc(4 * (size_t)(mc + 1));
+            for (int32_t a = 0; a < mc; ++a) rel[a] = front_local(t, J, Rc[a]);
+            const float* cb = CB + cb_off[C];
+            for (int32_t a = 0; a < mc; ++a)
+                for (int32_t b = 0; b < mc; ++b) F[rel[a] * r + rel[b]] += cb[a * mc + b];
+            for (int32_t a = 0; a < mc; ++a) fr[rel[a]] += RCB[rcb_off[C] + a];
+            free(rel);
+        }
+        for (int32_t k = 0; k < c; ++k) {
+            const float piv = F[k * r + k];
+            if (!(fabsf(piv) >= 1e-25f)) rc = NR_STATUS_SINGULAR;
+            for (int32_t i = k + 1; i < r; ++i) F[i * r + k] = F[i * r + k] / piv;
+            for (int32_t i = k + 1; i < r; ++i) {
+                const float l = F[i * r + k];
+                for (int32_t j = k + 1; j < r; ++j) F[i * r + j] = fmaf(-l, F[k * r + j], F[i * r + j]);
+                fr[i] = fmaf(-l, fr[k], fr[i]);
+            }
+        }
+        memcpy(U + u_off[J], F, 4 * (size_t)c * (size_t)r);
+        for (int32_t k = 0; k < c; ++k) y[f + k] = fr[k];
+        for (int32_t a = 0; a < m; ++a) {
+            for (int32_t b = 0; b < m; ++b) CB[cb_off[J] + a * m + b] = F[(c + a) * r + c + b];
+            RCB[rcb_off[J] + a] = fr[c + a];
+        }
+    }
+    if (rc == NR_OK) {
+        for (int32_t J = ns - 1; J >= 0; --J) {
+            const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+            const float* Uj = U + u_off[J];
+            const int32_t* R = t->sn_rows + t->sn_row_off[J];
+            for (int32_t k = 0; k < c; ++k) {
+                float s = y[f + k];
+                for (int32_t a = 0; a < m; ++a) s = fmaf(-Uj[k * r + c + a], xp[R[a]], s);
+                fr[k] = s;
+            }
+            for (int32_t k = c - 1; k >= 0; --k) {
+                const float xk = fr[k] / Uj[k * r + k];
+                xp[f + k] = xk;
+                for (int32_t i = 0; i < k; ++i) fr[i] = fmaf(-Uj[i * r + k], xk, fr[i]);
+            }
+        }
+        for (int32_t i = 0; i < n; ++i) x[t->perm[i]] = xp[i];
+    } else {
+        for (int32_t i = 0; i < n; ++i) x[i] = 0.0f;
+    }
+    free(u_off); free(cb_off); free(rcb_off); free(U); free(CB); free(RCB); free(y); free(xp); free(F); free(fr);
+    return rc;
+}
+
+static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v, double* res, const tran_ctx* tc);
+
+int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, double gmin, float* a32) {
+    orc_soa* s = orc_tensorize(nl);
+    double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* lin = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    int rc = assemble_f32(nl, s, t, v, gmin, NULL, vals, lin, a32);
+    free(vals); free(lin);
+    orc_soa_free(s);
+    return rc;
+}
+
+/* One Newton step of the FP32 path at v: Jacobian and LU in float, rhs =
+ * the FP64 Newton residual rounded to float, vn = v + the float correction
+ * (FP64 update).  The Newton residual b - A v of the reference's Norton
+ * system is the KCL audit vector minus the gmin diagonal current. */
+static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_topo* t, const double* v,
+                           double gmin, const tran_ctx* tc, double* vals, double* lin, float* a32, float* r32,
+                           float* x32, double* res, double* vn) {
+    const int32_t n = nl->num_nodes;
+    int rc = assemble_f32(nl, s, t, v, gmin, tc, vals, lin, a32);
+    if (rc != NR_OK) return rc;
+    kcl_soa(nl, s, v, res, tc);
+    for (int32_t i = 0; i < n; ++i) r32[i] = (float)(res[i] - gmin * v[i]);
+    rc = orc_lu_multifrontal_f32(t, a32, r32, x32);
+    for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + (double)x32[i];
+    return rc;
+}
+
 /* ------------------------------------------------------------------------ */
 /* a14: seedInitialGuess (solvers/circuitsim_solver.cpp:32-116)             */
 /* ------------------------------------------------------------------------ */
@@ -1188,7 +1364,7 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 /* ------------------------------------------------------------------------ */
 void orc_dc_opts_default(orc_dc_opts* o) {
     o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_MULTIFRONTAL;
-    o->start_step = 0; o->use_seed = 0;
+    o->start_step = 0; o->use_seed = 0; o->precision = ORC_PREC_FP64;
 }
 
 int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opts,
@@ -1207,6 +1383,10 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     double* res = (double*)xmalloc(sizeof(double) * (size_t)n);
     double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
     double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    const int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
+    float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
+    float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     if (o.use_seed) memcpy(v, v_io, sizeof(double) * (size_t)n);
     else orc_seed(nl, v);
     int converged = 0, singular = 0;
@@ -1218,12 +1398,17 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         int gconv = 0, sing_step = 0;
         int32_t it = 0;
         for (; it < o.max_iter; ++it) {
-            stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
-            orc_assemble(t, vals, av);
-            int rc = (o.lu_kind == ORC_LU_DENSE)
+            int rc;
+            if (p32) {
+                rc = newton_step_f32(nl, s, t, v, gmin, NULL, vals, av, a32, r32, x32, res, vn);
+            } else {
+                stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
+                orc_assemble(t, vals, av);
+                rc = (o.lu_kind == ORC_LU_DENSE)
                          ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
                          : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
                                                         : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+            }
             if (rc != NR_OK) {
                 final_kcl = 1e9;
                 singular = 1; sing_step = 1;
@@ -1265,7 +1450,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     out->gmin_steps_done = steps_done;
     out->kcl = fk;
     memcpy(v_io, v, sizeof(double) * (size_t)n);
-    free(v); free(vn); free(rhs); free(res); free(vals); free(av);
+    free(v); free(vn); free(rhs); free(res); free(vals); free(av); free(a32); free(r32); free(x32);
     orc_soa_free(s);
     if (own) orc_topology_free(own);
     return NR_OK;
@@ -1316,6 +1501,7 @@ double orc_pulse_value(const double* p, double time) {
 
 void orc_tran_opts_default(orc_tran_opts* o) {
     o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_MULTIFRONTAL;
+    o->precision = ORC_PREC_FP64;
 }
 
 int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_opts* opts, double* v_io,
@@ -1334,6 +1520,10 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
     double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
     double* vs = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_vsrc + 1));
     double* vprev = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    const int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
+    float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
+    float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     /* start state (D11): zero voltages and history, or the caller's state */
     if (o.start) memcpy(v, v_io, sizeof(double) * (size_t)n);
     else memset(v, 0, sizeof(double) * (size_t)n);
@@ -1353,12 +1543,17 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
         int conv = 0;
         int32_t it = 0;
         for (; it < o.max_iter; ++it) {
-            stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
-            orc_assemble(t, vals, av);
-            int rc = (o.lu_kind == ORC_LU_DENSE)
+            int rc;
+            if (p32) {
+                rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, vals, av, a32, r32, x32, res, vn);
+            } else {
+                stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
+                orc_assemble(t, vals, av);
+                rc = (o.lu_kind == ORC_LU_DENSE)
                          ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
                          : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
                                                         : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+            }
             if (rc != NR_OK) { status = NR_STATUS_SINGULAR; total += it + 1; kcl = 1e9; break; }
             for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
             kcl = kcl_soa(nl, s, vn, res, &tc);
@@ -1379,6 +1574,7 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
     out->kcl = kcl;
     memcpy(v_io, v, sizeof(double) * (size_t)n);
     free(v); free(vn); free(rhs); free(res); free(vals); free(av); free(vs); free(vprev);
+    free(a32); free(r32); free(x32);
     orc_soa_free(s);
     return NR_OK;
 }

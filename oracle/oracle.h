@@ -70,6 +70,10 @@ int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
  * The HIP library performs exactly these operations in this order. */
 int orc_lu_multifrontal(const orc_topo* t, const double* a_vals, const double* rhs,
                         double* x, double* smallest_pivot);
+/* The same in float (FP32 path). */
+int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float* rhs, float* x);
+/* FP32 path: the float Jacobian (CSR values) of the DC system at v. */
+int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, double gmin, float* a32);
 
 /* --- a14, a18, a19, a20 ---------------------------------------------------- */
 void orc_seed(const nr_netlist* nl, double* v);
@@ -84,12 +88,19 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 
 /* --- a15: solveDC ---------------------------------------------------------- */
 enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2 };
+/* ORC_PREC_FP64: the reference's FP64 path.  ORC_PREC_FP32_REFINE: the
+ * Jacobian (device evaluation, stamps, assembly) and the multifrontal LU in
+ * float; each Newton step solves J dv = b - A v with the Newton residual
+ * computed in FP64 and updates v in FP64; limiting, KCL and the convergence
+ * test stay FP64. */
+enum { ORC_PREC_FP64 = 0, ORC_PREC_FP32_REFINE = 1 };
 typedef struct orc_dc_opts {
     int32_t max_iter;     /* 50 */
     int32_t gmin_steps;   /* 19 */
-    int32_t lu_kind;      /* ORC_LU_* */
+    int32_t lu_kind;      /* ORC_LU_* (FP64 path) */
     int32_t start_step;   /* GMIN step to start at (seeded start) */
     int32_t use_seed;     /* 1: v_inout holds the start vector */
+    int32_t precision;    /* ORC_PREC_* */
 } orc_dc_opts;
 typedef struct orc_dc_out {
     int32_t iterations;      /* true NR iteration count */
@@ -118,7 +129,8 @@ typedef struct orc_tran_opts {
     int32_t steps;
     int32_t max_iter;     /* 50                                           */
     int32_t start;        /* 0: zero state (D11), 1: v_io holds the state */
-    int32_t lu_kind;      /* ORC_LU_*                                     */
+    int32_t lu_kind;      /* ORC_LU_* (FP64 path)                         */
+    int32_t precision;    /* ORC_PREC_*                                   */
 } orc_tran_opts;
 typedef struct orc_tran_out {
     int32_t iterations;   /* NR iterations over all steps                 */

@@ -22,12 +22,12 @@ _f64p = C.POINTER(C.c_double)
 
 class OrcDcOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("lu_kind", C.c_int32),
-                ("start_step", C.c_int32), ("use_seed", C.c_int32)]
+                ("start_step", C.c_int32), ("use_seed", C.c_int32), ("precision", C.c_int32)]
 
 
 class OrcTranOpts(C.Structure):
     _fields_ = [("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
-                ("lu_kind", C.c_int32)]
+                ("lu_kind", C.c_int32), ("precision", C.c_int32)]
 
 
 class OrcTranOut(C.Structure):
@@ -43,6 +43,9 @@ class OrcDcOut(C.Structure):
 LU_DENSE = 0
 LU_SPARSE = 1
 LU_MULTIFRONTAL = 2
+
+PREC_FP64 = 0
+PREC_FP32_REFINE = 1
 
 
 def lib():
@@ -128,23 +131,24 @@ class Topology:
             self.h = None
 
 
-def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0):
-    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed)
+def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0, precision=PREC_FP64):
+    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision)
 
 
 def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, gmin_steps=19,
-             seed_v=None, start_step=0):
+             seed_v=None, start_step=0, precision=PREC_FP64):
     c, keep = nl.to_c()
     v = np.zeros(nl.num_nodes) if seed_v is None else np.array(seed_v, dtype=np.float64)
     out = OrcDcOut()
-    o = dc_opts(max_iter, gmin_steps, lu, start_step, 0 if seed_v is None else 1)
+    o = dc_opts(max_iter, gmin_steps, lu, start_step, 0 if seed_v is None else 1, precision)
     lib().orc_solve_dc(C.byref(c), topo.h if topo else None, C.byref(o), _p64(v), C.byref(out))
     return v, out
 
 
 def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL, threads=0,
-                   max_iter=50, gmin_steps=19):
-    """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array."""
+                   max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0):
+    """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array;
+    seeds (optional, [B, n]) start every instance at GMIN step start_step."""
     c, keep = nl.to_c()
     arrs = (_f64p * _abi.N_FIELDS)()
     hold = []
@@ -155,9 +159,10 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL,
             arrs[fid] = _p64(a)
         else:
             arrs[fid] = None
-    v = np.zeros((B, nl.num_nodes))
+    v = np.zeros((B, nl.num_nodes)) if seeds is None else np.array(seeds, dtype=np.float64).reshape(B, -1)
     outs = (OrcDcOut * B)()
-    o = dc_opts(max_iter, gmin_steps, lu)
+    o = dc_opts(max_iter, gmin_steps, lu, start_step if seeds is not None else 0, 0 if seeds is None else 1,
+                precision)
     lib().orc_solve_dc_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), outs)
     stats = {
         "iterations": np.array([x.iterations for x in outs]),
@@ -175,7 +180,7 @@ def pulse_value(p, t):
 
 
 def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, steps=200, max_iter=50, start=0,
-                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0):
+                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0, precision=PREC_FP64):
     """Backward-Euler transient of B instances; returns (v_final [B, n],
     wave [B, steps+1, n_probe], stats)."""
     c, keep = nl.to_c()
@@ -192,7 +197,7 @@ def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, step
     pr = np.ascontiguousarray(probes if probes is not None else [], np.int32)
     wave = np.zeros((B, steps + 1, len(pr)))
     outs = (OrcTranOut * B)()
-    o = OrcTranOpts(h, steps, max_iter, start, lu)
+    o = OrcTranOpts(h, steps, max_iter, start, lu, precision)
     lib().orc_solve_transient_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), len(pr),
                                     _p32(pr) if len(pr) else None, _p64(wave), outs)
     stats = {"iterations": np.array([x.iterations for x in outs]), "status": np.array([x.status for x in outs]),
@@ -266,6 +271,30 @@ def lu_multifrontal(topo: Topology, a_vals, rhs):
     rc = L.orc_lu_multifrontal(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
                                _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(sp))
     return rc, x
+
+
+def lu_multifrontal_f32(topo: Topology, a_vals, rhs):
+    L = lib()
+    fp = C.POINTER(C.c_float)
+    L.orc_lu_multifrontal_f32.argtypes = [C.c_void_p, fp, fp, fp]
+    a = np.ascontiguousarray(a_vals, np.float32)
+    r = np.ascontiguousarray(rhs, np.float32)
+    x = np.zeros(topo.n, np.float32)
+    rc = L.orc_lu_multifrontal_f32(topo.h, a.ctypes.data_as(fp), r.ctypes.data_as(fp), x.ctypes.data_as(fp))
+    return rc, x
+
+
+def jacobian_f32(nl, topo: Topology, v, gmin):
+    """FP32 path: the float Jacobian (CSR values) of the DC system at v."""
+    L = lib()
+    fp = C.POINTER(C.c_float)
+    L.orc_jacobian_f32.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, C.c_double, fp]
+    c, keep = nl.to_c()
+    v = np.ascontiguousarray(v, np.float64)
+    a = np.zeros(topo.nnz_a, np.float32)
+    if L.orc_jacobian_f32(C.byref(c), topo.h, _p64(v), gmin, a.ctypes.data_as(fp)):
+        raise RuntimeError("orc_jacobian_f32 failed")
+    return a
 
 
 def seed(nl):

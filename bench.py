@@ -25,7 +25,6 @@ from __future__ import annotations
 import argparse
 import json
 import os
-import re
 import sys
 import time
 
@@ -58,14 +57,13 @@ def copy_probe(gib=2):
 
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
-    (lines `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+    (lines `traffic_bytes_per_launch | <kernel> | <bytes>`), or None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None
-    pat = re.compile(r"^traffic_bytes_per_launch\s+(\S+)\s+([0-9.eE+]+)")
     for ln in open(TRAFFIC_FILE):
-        m = pat.match(ln.strip().strip("`"))
-        if m and m.group(1) == kernel:
-            return float(m.group(2))
+        f = [x.strip() for x in ln.split("|")]
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return float(f[2])
     return None
 
 

@@ -211,7 +211,9 @@ typedef struct nr_solve_opts {
     int32_t seeded_start_step; /* GMIN step where seeded instances start      */
     int32_t record_timing;   /* 1: per-phase HIP event timing; 2: also per
                                 kernel launch (nr_batch_get_kernel_stats)    */
-    int32_t use_graph;       /* 1: replay each batched iteration as a hipGraph*/
+    int32_t use_graph;       /* 1: replay each batched iteration as a hipGraph
+                                (captured once per grid size and argument
+                                set; per-kernel timing is then not kept)   */
     int32_t fallback_plain;  /* 1: an instance whose seeded solve does not
                                 converge is solved again from the plain
                                 GMIN-stepping start                           */
@@ -258,6 +260,7 @@ typedef struct nr_transient_opts {
     const int32_t* probe;    /* 1-based node ids, n_probe of them            */
     int32_t record_timing;   /* as in nr_solve_opts                          */
     int32_t precision;       /* NR_PREC_*, as in nr_solve_opts               */
+    int32_t use_graph;       /* as in nr_solve_opts                          */
 } nr_transient_opts;
 NR_API void nr_transient_opts_default(nr_transient_opts* opts);
 

@@ -106,7 +106,7 @@ class NrTransientOpts(C.Structure):
     _fields_ = [
         ("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
         ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)), ("record_timing", C.c_int32),
-        ("precision", C.c_int32),
+        ("precision", C.c_int32), ("use_graph", C.c_int32),
     ]
 
 

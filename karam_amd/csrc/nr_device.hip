@@ -110,7 +110,7 @@ struct DevTopo {
     // parent's pivot columns (a prefix, the maps are increasing), and the
     // inverse map parent-local row -> child row (or -1), per child
     const int *sn_pc, *inv_off, *inv;
-    long long uS, uL, cbL, rcbL, nAl, zero_item;
+    long long zero_item;
     // transient: capacitors and PULSE sources (v1 v2 td tr tf pw per)
     int nC;
     const int *cap_n1, *cap_n2, *vsrc_is_pulse;
@@ -134,9 +134,11 @@ struct DevBatch {
     // interleaved evaluation / assembly state
     double *vI, *glin, *rhslin;
     double *dg, *dieq, *mgds, *mgm, *mieq;
-    // small-front side (interleaved) and large-front side (instance-major)
+    // small-front pool and factors (interleaved), the large side's A entries
+    // and rhs (interleaved) and front blocks (front-major, see fblk); ys / xs
+    // hold the forward and backward solutions of every pivot (interleaved)
     double *pool, *Us, *ys, *xs;
-    double *Al, *rhsl, *UL, *CBL, *RCBL, *yl, *xl;
+    double *Al, *rhsl, *UL, *CBL, *RCBL;
     double* seed;                // optional [B][n]
     // transient: capacitances, capacitor voltages of the last accepted step,
     // source values at the current time, the step h; vnow = the source values
@@ -484,6 +486,7 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         s.prev_kcl = 0.0; s.kcl = 0.0;
         P.st[b] = s;
         P.act[b] = b;
+        if (b == 0) *P.nact = P.B;
     }
 }
 
@@ -542,8 +545,9 @@ __global__ __launch_bounds__(64) void k_wave_start(DevTopo T, DevBatch P, int st
     const int b = act[slot_];                                             \
     const int Bp = P.Bp;
 
-__global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+__global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     LANE_SETUP(4)
     const int n = T.n;
     auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
@@ -571,9 +575,10 @@ __global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* 
 // A = linear part + device stamps (emission order) + gmin; rhs likewise.
 // Each value goes to the front that owns it: the interleaved small-front pool
 // or the instance-major large-front arrays.
-__global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+__global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                   const double* __restrict__ gmin_tab, int gmin_last) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     LANE_SETUP(4)
     if (item < T.nnz_a) {
         double a = P.glin[IX(item, b, Bp)];
@@ -590,7 +595,7 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
         }
         if (T.slot_diag[item]) a = a + gmin_tab[min(P.st[b].step, gmin_last)];
         const int d = T.slot_dst[item];
-        if (d & 1) P.Al[(size_t)b * T.nAl + (d >> 1)] = a;
+        if (d & 1) P.Al[IX(d >> 1, b, Bp)] = a;
         else P.pool[IX(d >> 1, b, Bp)] = a;
     }
     if (item < T.n) {
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
             r = term_neg(t) ? r + (-val) : r + val;
         }
         const int d = T.node_dst[item];
-        if (d & 1) P.rhsl[(size_t)b * T.n + (d >> 1)] = r;
+        if (d & 1) P.rhsl[IX(d >> 1, b, Bp)] = r;
         else P.pool[IX(d >> 1, b, Bp)] = r;
     }
 }
@@ -639,8 +644,9 @@ __device__ __forceinline__ void mos_g32(double vgs64, double vds64, double W, do
     if (gds < 1e-12f) gds = 1e-12f;
 }
 
-__global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+__global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     LANE_SETUP(4)
     const int n = T.n;
     auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
@@ -662,9 +668,10 @@ __global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int
 // FP32 path: A in float (the linear part, summed in FP64, rounded once; then
 // the FP32 device stamps and the gmin diagonal in emission order) and the
 // Newton rhs b - A v = KCL vector - gmin v, formed in FP64 and rounded
-__global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+__global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                     const double* __restrict__ gmin_tab, int gmin_last) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     LANE_SETUP(4)
     float* pool = reinterpret_cast<float*>(P.pool);
     const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
@@ -686,13 +693,13 @@ __global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const
         }
         if (T.slot_diag[item]) a = a + (float)gmin;
         const int d = T.slot_dst[item];
-        if (d & 1) reinterpret_cast<float*>(P.Al)[(size_t)b * T.nAl + (d >> 1)] = a;
+        if (d & 1) reinterpret_cast<float*>(P.Al)[IX(d >> 1, b, Bp)] = a;
         else pool[IX(d >> 1, b, Bp)] = a;
     }
     if (item < T.n) {
         const float r = (float)(P.rhslin[IX(item, b, Bp)] - gmin * P.vI[IX(item, b, Bp)]);
         const int d = T.node_dst[item];
-        if (d & 1) reinterpret_cast<float*>(P.rhsl)[(size_t)b * T.n + (d >> 1)] = r;
+        if (d & 1) reinterpret_cast<float*>(P.rhsl)[IX(d >> 1, b, Bp)] = r;
         else pool[IX(d >> 1, b, Bp)] = r;
     }
 }
@@ -706,6 +713,12 @@ __device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
 // the factor buffers are sized for FP64; the FP32 path uses their first half
 template <typename V>
 __device__ __forceinline__ V* vp(double* p) { return reinterpret_cast<V*>(p); }
+// Front blocks (pivot rows, update blocks and their rhs parts) of the large
+// side are stored front-major: front J's blocks for instances 0..B-1 lie
+// side by side, so the teams of one launch touch one contiguous region.
+__device__ __forceinline__ size_t fblk(long long off, int b, int size, int B) {
+    return (size_t)off * (size_t)B + (size_t)b * (size_t)size;
+}
 
 // ---------------------------------------------------------------------------
 // kernels: small fronts (lane = instance, front in VGPRs)
@@ -758,8 +771,8 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
     if (on) {
         const long long uo = T.sn_u_off[J], co = T.sn_cb_off[J], ro = T.sn_rcb_off[J];
         const bool root = T.sn_root[J] != 0;
-        V* cbl = vp<V>(P.CBL) + (size_t)b * T.cbL + co;
-        V* rcbl = vp<V>(P.RCBL) + (size_t)b * T.rcbL + ro;
+        V* cbl = vp<V>(P.CBL) + fblk(co, b, m * m, P.B);
+        V* rcbl = vp<V>(P.RCBL) + fblk(ro, b, m, P.B);
 #pragma unroll
         for (int i = 0; i < R; ++i) {
             if (i < c) {
@@ -815,9 +828,10 @@ __device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, 
 // Every front of the subtrees in one launch is padded to RMAX, so each
 // kernel carries a single front shape (register budget and code size).
 template <int RMAX, typename V>
-__global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+__global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                       const int* __restrict__ sub_ptr,
                                                       const int* __restrict__ sub_sn, int n_sub) {
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     const int lane = threadIdx.x & 63;
     const long long task = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int ngroups = (nact + 63) / 64;
@@ -833,8 +847,9 @@ __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, con
 
 template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, const int* __restrict__ act,
-                                                         int nact, const int* __restrict__ sub_ptr,
+                                                         int nact_grid, const int* __restrict__ sub_ptr,
                                                          const int* __restrict__ sub_sn, int n_sub) {
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     const int lane = threadIdx.x & 63;
     const long long task = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int ngroups = (nact + 63) / 64;
@@ -877,9 +892,10 @@ __device__ __forceinline__ void team_sync() {
 //       gathered straight from the children's update blocks (every A entry of
 //       a front lies in a pivot row or column, so only children contribute).
 template <int TEAM, int NT, bool FULL, typename V>
-__global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact,
+__global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                      const int* __restrict__ sn_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
@@ -907,7 +923,8 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     for (int e = lane; e < np; e += TEAM) Pr[e] = V(0);
     team_sync<TEAM>();
     // (A) assembly: A entries, the pivot rows' rhs, then each child in order
-    const V* Ab = vp<V>(P.Al) + (size_t)b * T.nAl;
+    const int Bp = P.Bp;
+    const V* Al = vp<V>(P.Al);
     {
         const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
         for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
@@ -915,7 +932,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 const int e = e0 + u * TEAM;
-                vals[u] = (e < a1) ? Ab[T.fa_slot[e]] : V(0);
+                vals[u] = (e < a1) ? Al[IX(T.fa_slot[e], b, Bp)] : V(0);
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
@@ -924,16 +941,16 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
             }
         }
     }
-    const V* rb = vp<V>(P.rhsl) + (size_t)b * T.n;
-    for (int k = lane; k < c; k += TEAM) fr[k] += rb[T.perm[f + k]];
+    const V* rl = vp<V>(P.rhsl);
+    for (int k = lane; k < c; k += TEAM) fr[k] += rl[IX(T.perm[f + k], b, Bp)];
     team_sync<TEAM>();
     const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
     for (int ch = ch0; ch < ch1; ++ch) {
         const int C = T.children[ch];
         const int mc = T.sn_nrows[C], pc = FULL ? mc : T.sn_pc[C];
         const int* rel = T.relmap + T.sn_row_off[C];
-        const V* cb = vp<V>(P.CBL) + (size_t)b * T.cbL + T.sn_cb_off[C];
-        const V* rcb = vp<V>(P.RCBL) + (size_t)b * T.rcbL + T.sn_rcb_off[C];
+        const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
+        const V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[C], b, mc, P.B);
         // FULL: every entry; else rows a < pc (all columns) and columns q < pc
         // of the remaining rows (the child's entries that land in the panel)
         const int n1 = pc * mc, nn = n1 + (mc - pc) * pc;
@@ -1008,8 +1025,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     team_sync<TEAM>();
     // (D) trailing block in 4x4 register tiles, streamed to the update block
     if (m > 0) {
-        V* cbJ = vp<V>(P.CBL) + (size_t)b * T.cbL + T.sn_cb_off[J];
-        const V* cbase = vp<V>(P.CBL) + (size_t)b * T.cbL;
+        V* cbJ = vp<V>(P.CBL) + fblk(T.sn_cb_off[J], b, m * m, P.B);
         const int mt = (m + 3) >> 2;
         for (int tt = lane; tt < mt * mt; tt += TEAM) {
             const int i0 = (tt / mt) * 4, j0 = (tt % mt) * 4;
@@ -1029,7 +1045,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                     const int C = T.children[ch];
                     const int mc = T.sn_nrows[C];
                     const int* inv = T.inv + T.inv_off[C] + c;
-                    const V* cb = cbase + T.sn_cb_off[C];
+                    const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
                     int ra[4], qa[4];
 #pragma unroll
                     for (int a = 0; a < 4; ++a) {
@@ -1066,25 +1082,25 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 for (int q = 0; q < 4; ++q)
                     if (i0 + a < m && j0 + q < m) cbJ[(i0 + a) * m + j0 + q] = acc[a][q];
         }
-        V* rcb = vp<V>(P.RCBL) + (size_t)b * T.rcbL + T.sn_rcb_off[J];
+        V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[J], b, m, P.B);
         for (int i = lane; i < m; i += TEAM) {
             V acc = fr[c + i];
             for (int k = 0; k < c; ++k) acc = vfma(-Lc[i * ls + k], fr[k], acc);
             rcb[i] = acc;
         }
     }
-    V* U = vp<V>(P.UL) + (size_t)b * T.uL + T.sn_u_off[J];
+    V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
     for (int e = lane; e < c * r; e += TEAM) U[e] = Pr[e];
-    V* yb = vp<V>(P.yl) + (size_t)b * T.n;
-    for (int k = lane; k < c; k += TEAM) yb[f + k] = fr[k];
+    for (int k = lane; k < c; k += TEAM) vp<V>(P.ys)[IX(f + k, b, Bp)] = fr[k];
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
 template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
-                                                        int nact, const int* __restrict__ sn_list, int n_sn,
+                                                        int nact_grid, const int* __restrict__ sn_list, int n_sn,
                                                         int rmax) {
 #pragma clang fp contract(off)
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
@@ -1103,14 +1119,14 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     const int Bp = P.Bp;
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
     V* xl = smem + (size_t)tib * (size_t)rmax;
-    const V* U = vp<V>(P.UL) + (size_t)b * T.uL + T.sn_u_off[J];
+    const V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
     const int* rows = T.row_idx + T.sn_row_off[J];
-    V* xb = vp<V>(P.xl) + (size_t)b * T.n;
-    const V* yb = vp<V>(P.yl) + (size_t)b * T.n;
-    for (int a = lane; a < m; a += TEAM) xl[c + a] = xb[rows[a]];
+    V* xs = vp<V>(P.xs);
+    const V* ys = vp<V>(P.ys);
+    for (int a = lane; a < m; a += TEAM) xl[c + a] = xs[IX(rows[a], b, Bp)];
     team_sync<TEAM>();
     for (int k = lane; k < c; k += TEAM) {
-        V s = yb[f + k];
+        V s = ys[IX(f + k, b, Bp)];
         const V* Uk = U + (size_t)k * r + c;
         for (int a = 0; a < m; ++a) s = vfma(-Uk[a], xl[c + a], s);
         xl[k] = s;
@@ -1119,10 +1135,7 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     for (int k = c - 1; k >= 0; --k) {
         const V xk = xl[k] / U[(size_t)k * r + k];
         for (int i = lane; i < k; i += TEAM) xl[i] = vfma(-U[(size_t)i * r + k], xk, xl[i]);
-        if (lane == 0) {
-            xb[f + k] = xk;
-            vp<V>(P.xs)[IX(f + k, b, Bp)] = xk;
-        }
+        if (lane == 0) xs[IX(f + k, b, Bp)] = xk;
         team_sync<TEAM>();
     }
 }
@@ -1132,7 +1145,8 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
 // ---------------------------------------------------------------------------
 // xs (interleaved, permuted rows) -> xm (instance-major, permuted rows, FP64)
 template <typename V>
-__global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+__global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64, g = blockIdx.y;
@@ -1155,7 +1169,8 @@ __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const in
 
 // v (instance-major) -> vI (interleaved); on the FP32 path also the KCL
 // vector res -> rhslin
-__global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+__global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
+    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64, g = blockIdx.y;
@@ -1194,6 +1209,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
 #pragma clang fp contract(off)
     __shared__ double red[8];
     __shared__ int redi[8];
+    if ((int)blockIdx.x >= *P.nact) return;
     const int b = act[blockIdx.x];
     const int n = T.n;
     InstState st = P.st[b];

@@ -293,7 +293,7 @@ class Batch:
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
 
     def transient(self, h=1e-9, steps=200, max_iter=50, start=_abi.NR_TRAN_START_ZERO, probes=None,
-                  record_timing=0, precision=_abi.NR_PREC_FP64):
+                  record_timing=0, precision=_abi.NR_PREC_FP64, use_graph=0):
         """Fixed-step backward-Euler transient of every instance (needs a
         transient topology).  probes: 1-based node ids recorded each step."""
         o = _abi.NrTransientOpts()
@@ -304,6 +304,7 @@ class Batch:
         o.probe = _p32(self._probes) if len(self._probes) else None
         o.record_timing = record_timing
         o.precision = int(precision)
+        o.use_graph = int(use_graph)
         r = _abi.NrResult()
         _check(lib().nr_batch_transient(self.h, C.byref(o), C.byref(r)), self.ctx)
         self._wave_shape = (self.B, int(steps) + 1, len(self._probes))

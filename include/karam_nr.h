@@ -264,6 +264,30 @@ typedef struct nr_transient_opts {
 } nr_transient_opts;
 NR_API void nr_transient_opts_default(nr_transient_opts* opts);
 
+/* ---- autotuner ------------------------------------------------------------ */
+/* No counterpart in the reference (its solves run on the CPU).  Large fronts
+ * (order > 8) are factored by one team of lanes per (front, instance); this
+ * chooses the team size and threads per block for each bucket of front order
+ * by timing the candidate launch shapes on the batch itself (its current
+ * parameters; factorisation + back substitution of every instance, `reps`
+ * times per candidate, one bucket after another).  The choice is stored in
+ * the batch's topology, for the given precision, and used by every later
+ * solve of batches of that topology.  It does not change any result: the
+ * elimination performs the same operations in the same order whatever the
+ * team size.  Overwrites the batch's solution state; not to be run while a
+ * batch of the same topology is solving. */
+enum { NR_TUNE_BUCKETS = 8 };
+typedef struct nr_autotune_result {
+    int32_t n_buckets;
+    int32_t bucket_rmax[NR_TUNE_BUCKETS];  /* largest front order in the bucket  */
+    int32_t team[NR_TUNE_BUCKETS];         /* chosen lanes per (front, instance) */
+    int32_t block[NR_TUNE_BUCKETS];        /* chosen threads per block           */
+    int32_t fronts[NR_TUNE_BUCKETS];       /* large fronts in the bucket         */
+    double ms_before, ms_after;            /* factor + back substitution of the batch */
+} nr_autotune_result;
+NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
+                             nr_autotune_result* out);
+
 /* Fixed-step backward-Euler transient of every instance (the restated
  * stepTransient: companion G = C/h with history (C/h) v_prev, PULSE sources
  * at t_{n+1}, Newton from the previous step's solution with the 1e-12

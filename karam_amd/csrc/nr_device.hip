@@ -45,6 +45,7 @@
 #include <memory>
 #include <random>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "karam_nr.h"
@@ -895,6 +896,9 @@ template <int TEAM, int NT, bool FULL, typename V>
 __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                      const int* __restrict__ sn_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
+    // a team wider than a wavefront synchronises the whole block, so it must
+    // be alone in it (early exits and per-front barrier counts then agree)
+    static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
     const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);
@@ -1100,6 +1104,9 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
                                                         int nact_grid, const int* __restrict__ sn_list, int n_sn,
                                                         int rmax) {
 #pragma clang fp contract(off)
+    // a team wider than a wavefront synchronises the whole block, so it must
+    // be alone in it (early exits and per-front barrier counts then agree)
+    static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
     const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);

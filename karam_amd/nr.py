@@ -39,7 +39,7 @@ EXPORTED = [
     "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
     "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
     "nr_eval_diodes", "nr_eval_mosfets",
-    "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform",
+    "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform", "nr_batch_autotune",
     "nr_seeder_create", "nr_seeder_destroy", "nr_seeder_record", "nr_seeder_history", "nr_seeder_reset",
     "nr_train_opts_default", "nr_seeder_train", "nr_seeder_predict", "nr_batch_set_seed_from",
     "nr_seeder_param_count", "nr_seeder_get_params", "nr_seeder_set_params", "nr_seeder_loss_grad",
@@ -66,6 +66,7 @@ def lib():
         L.nr_transient_opts_default.argtypes = [C.POINTER(_abi.NrTransientOpts)]
         L.nr_transient_opts_default.restype = None
         L.nr_batch_transient.argtypes = [_vp, C.POINTER(_abi.NrTransientOpts), C.POINTER(_abi.NrResult)]
+        L.nr_batch_autotune.argtypes = [_vp, C.c_int32, C.c_int32, C.POINTER(_abi.NrAutotuneResult)]
         L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
         _f32p = C.POINTER(C.c_float)
         L.nr_seeder_create.argtypes = [_vp, C.POINTER(_abi.NrSeederCfg), C.POINTER(_vp)]
@@ -309,6 +310,17 @@ class Batch:
         _check(lib().nr_batch_transient(self.h, C.byref(o), C.byref(r)), self.ctx)
         self._wave_shape = (self.B, int(steps) + 1, len(self._probes))
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
+        """Choose the large-front launch shape per front-order bucket by timing
+        the candidates on this batch (nr_batch_autotune); the choice is kept
+        by the topology.  Returns the chosen map and the factor + back
+        substitution time before and after."""
+        r = _abi.NrAutotuneResult()
+        _check(lib().nr_batch_autotune(self.h, int(precision), int(reps), C.byref(r)), self.ctx)
+        return {"buckets": [{"rmax": r.bucket_rmax[b], "team": r.team[b], "block": r.block[b], "fronts": r.fronts[b]}
+                            for b in range(r.n_buckets)],
+                "ms_before": r.ms_before, "ms_after": r.ms_after}
 
     def waveform(self):
         """Probe voltages of the last transient, [B, steps + 1, n_probe]."""

@@ -11,6 +11,8 @@ HBM.  Prints ONE JSON line:
   fp32_refine  the same solves on the FP32 path (FP32 Jacobian and LU, FP64
                Newton residual and update), with its error against the FP64
                results
+  autotune     the launch shapes chosen per front-order bucket (untimed,
+               before the timed solves) and the factor time they gave
   seeded       learned-seeder starts on the FP64 and on the FP32 path; the
                FP32 one's speedup over plain FP64 is `target_speedup`
   cpu_baseline the oracle (FP64 C restatement, OpenMP over instances) timed
@@ -306,6 +308,7 @@ def main():
     ap.add_argument("--logic-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe()
@@ -329,6 +332,16 @@ def main():
     bt = nr.Batch(ctx, T, B)
     bt.set_fields(fields)
     log(f"workload rdmesh{a.side} B={B} n={T.info['n']} nnz_f={T.info['nnz_f']} symbolic {symbolic_ms:.1f} ms")
+    tune = {}
+    if not a.no_tune:                 # untimed: launch shapes per front-order bucket, per precision
+        for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
+            t0 = time.perf_counter()
+            tu = bt.autotune(precision=prec, reps=3)
+            tune[key] = {"s": time.perf_counter() - t0, "factor_solve_ms_before": tu["ms_before"],
+                         "factor_solve_ms_after": tu["ms_after"],
+                         "buckets": [[b["rmax"] if b["rmax"] < 1 << 20 else None, b["team"], b["block"], b["fronts"]]
+                                     for b in tu["buckets"]]}
+            log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution")
     for w in range(a.warmup):
         r = bt.solve_dc()
         log(f"warmup {w}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
@@ -354,6 +367,7 @@ def main():
         "roofline": roofline(kst),
         "kernels": kernel_table(kst),
         "cpu_baseline": None,
+        "autotune": tune or None,
     }
     # the FP32 path on the same instances, its error against the FP64 results
     bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)                # warm-up

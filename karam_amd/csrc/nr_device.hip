@@ -111,6 +111,8 @@ struct DevTopo {
     // parent's pivot columns (a prefix, the maps are increasing), and the
     // inverse map parent-local row -> child row (or -1), per child
     const int *sn_pc, *inv_off, *inv;
+    // fronts whose panel is kept in HBM: offset of their panel (front-major)
+    const long long* sn_gp_off;
     long long zero_item;
     // transient: capacitors and PULSE sources (v1 v2 td tr tf pw per)
     int nC;
@@ -140,6 +142,7 @@ struct DevBatch {
     // hold the forward and backward solutions of every pivot (interleaved)
     double *pool, *Us, *ys, *xs;
     double *Al, *rhsl, *UL, *CBL, *RCBL;
+    double* gpanel;              // HBM panels of the fronts too large for LDS
     double* seed;                // optional [B][n]
     // transient: capacitances, capacitor voltages of the last accepted step,
     // source values at the current time, the step h; vnow = the source values
@@ -878,21 +881,22 @@ __device__ __forceinline__ void team_sync() {
     }
 }
 
-// Large fronts, one team per (front, instance).  The front is assembled in
-// LDS (in full when FULL, else only its panel: the c pivot rows (c x r), the
-// pivot columns below them (m x c) and the rhs), then eliminated in three
-// phases that reproduce, element by element, the operations and their order
-// of a right-looking elimination of the assembled front:
+// Large fronts, one team per (front, instance).  The front's panel (the c
+// pivot rows (c x r), the pivot columns below them (m x c) and the rhs) is
+// assembled in LDS, or with GP in a scratch area in HBM (fronts whose panel
+// exceeds the LDS a block holds), then eliminated in three phases that
+// reproduce, element by element, the operations and their order of a
+// right-looking elimination of the assembled front:
 //   (B) one wavefront factors the c x c diagonal block and the pivot rows'
 //       rhs, with wavefront-level barriers only;
 //   (C) every lane takes a column of the pivot rows right of the block (U12)
 //       or a row of the pivot columns below it (L21) and applies the c
 //       updates to it alone: no barrier;
 //   (D) the trailing m x m block (the front's update block) is formed in
-//       4x4 register tiles and streamed to HBM.  Without FULL its entries are
-//       gathered straight from the children's update blocks (every A entry of
-//       a front lies in a pivot row or column, so only children contribute).
-template <int TEAM, int NT, bool FULL, typename V>
+//       4x4 register tiles and streamed to HBM, its entries gathered straight
+//       from the children's update blocks (every A entry of a front lies in a
+//       pivot row or column, so only children contribute).
+template <int TEAM, int NT, bool GP, typename V>
 __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                      const int* __restrict__ sn_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
@@ -916,14 +920,15 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     if (slot >= nact) return;
     const int b = act[slot];
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    V* Pr = smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
+    const int np = c * r + m * c + r;                 // panel size
+    V* Pr = GP ? vp<V>(P.gpanel) + fblk(T.sn_gp_off[J], b, np, P.B)
+               : smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
     // front-local (i, j) with i < c or j < c; rows c.. of the pivot columns
-    // have stride r in the full front, c in the panel
-    const int ls = FULL ? r : c;
-    V* Lc = FULL ? Pr + c * r : Pr + c * r;           // row i of L21 at Lc + i * ls
-    V* fr = FULL ? Pr + r * r : Pr + c * r + m * c;   // rhs, r
+    // have stride c in the panel
+    const int ls = c;
+    V* Lc = Pr + c * r;                               // row i of L21 at Lc + i * ls
+    V* fr = Pr + c * r + m * c;                       // rhs, r
     auto pidx = [&](int i, int j) { return i < c ? i * r + j : c * r + (i - c) * ls + j; };
-    const int np = FULL ? r * r + r : c * r + m * c + r;
     for (int e = lane; e < np; e += TEAM) Pr[e] = V(0);
     team_sync<TEAM>();
     // (A) assembly: A entries, the pivot rows' rhs, then each child in order
@@ -951,12 +956,12 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
     for (int ch = ch0; ch < ch1; ++ch) {
         const int C = T.children[ch];
-        const int mc = T.sn_nrows[C], pc = FULL ? mc : T.sn_pc[C];
+        const int mc = T.sn_nrows[C], pc = T.sn_pc[C];
         const int* rel = T.relmap + T.sn_row_off[C];
         const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
         const V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[C], b, mc, P.B);
-        // FULL: every entry; else rows a < pc (all columns) and columns q < pc
-        // of the remaining rows (the child's entries that land in the panel)
+        // rows a < pc (all columns) and columns q < pc of the remaining rows:
+        // the child's entries that land in the panel
         const int n1 = pc * mc, nn = n1 + (mc - pc) * pc;
         for (int e0 = lane; e0 < nn; e0 += 8 * TEAM) {
             V vals[8];
@@ -968,7 +973,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 if (e < n1) { a = e / mc; q = e - a * mc; }
                 else if (e < nn) { const int e2 = e - n1; a = pc + e2 / pc; q = e2 - (a - pc) * pc; }
                 vals[u] = (e < nn) ? cb[a * mc + q] : V(0);
-                pos[u] = (e < nn) ? (FULL ? rel[a] * r + rel[q] : pidx(rel[a], rel[q])) : 0;
+                pos[u] = (e < nn) ? pidx(rel[a], rel[q]) : 0;
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u)
@@ -1034,40 +1039,32 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
         for (int tt = lane; tt < mt * mt; tt += TEAM) {
             const int i0 = (tt / mt) * 4, j0 = (tt % mt) * 4;
             V acc[4][4];
-            if constexpr (FULL) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[a][q] = V(0);
+            for (int ch = ch0; ch < ch1; ++ch) {
+                const int C = T.children[ch];
+                const int mc = T.sn_nrows[C];
+                const int* inv = T.inv + T.inv_off[C] + c;
+                const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
+                int ra[4], qa[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
+                    qa[a] = (j0 + a < m) ? inv[j0 + a] : -1;
+                }
+                V x[4][4];
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
                     for (int q = 0; q < 4; ++q)
-                        acc[a][q] = (i0 + a < m && j0 + q < m) ? Pr[(c + i0 + a) * r + c + j0 + q] : V(0);
-            } else {
+                        x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : V(0);
 #pragma unroll
                 for (int a = 0; a < 4; ++a)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[a][q] = V(0);
-                for (int ch = ch0; ch < ch1; ++ch) {
-                    const int C = T.children[ch];
-                    const int mc = T.sn_nrows[C];
-                    const int* inv = T.inv + T.inv_off[C] + c;
-                    const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
-                    int ra[4], qa[4];
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
-                        qa[a] = (j0 + a < m) ? inv[j0 + a] : -1;
-                    }
-                    V x[4][4];
-#pragma unroll
-                    for (int a = 0; a < 4; ++a)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : V(0);
-#pragma unroll
-                    for (int a = 0; a < 4; ++a)
-#pragma unroll
-                        for (int q = 0; q < 4; ++q)
-                            if (ra[a] >= 0 && qa[q] >= 0) acc[a][q] += x[a][q];
-                }
+                    for (int q = 0; q < 4; ++q)
+                        if (ra[a] >= 0 && qa[q] >= 0) acc[a][q] += x[a][q];
             }
             for (int k = 0; k < c; ++k) {
                 V l[4], u[4];

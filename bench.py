@@ -291,6 +291,92 @@ def logic_workload(a):
     return 0
 
 
+def mixed_workload(a):
+    """Uneven sizes: a.mixed_instances instances whose node counts are
+    log-uniform in [200, 50000], the family of instance i being i mod 3 (the
+    R-diode mesh, the inverter chain, random inverter logic).  Node counts are
+    bucketed into 8 log-spaced sizes per family; each bucket is one circuit
+    with its own sparsity pattern and ordering, solved as one batch of
+    Monte-Carlo variants: DC operating points on the FP32 path with hipGraph
+    sweeps, launch shapes autotuned per circuit (untimed).  Symbolic analysis
+    is reported separately."""
+    from karam_amd import _abi, netlists, nr
+    rng = np.random.default_rng(12345)
+    N = a.mixed_instances
+    n_of = np.exp(rng.uniform(np.log(200.0), np.log(50000.0), N))
+    fam_of = np.arange(N) % 3
+    edges = np.exp(np.linspace(np.log(200.0), np.log(50000.0), 9))
+    bk = np.clip(np.searchsorted(edges, n_of, side="right") - 1, 0, 7)
+    ctx = nr.Context(0)
+    prec = _abi.NR_PREC_FP32_REFINE if a.mixed_precision == "fp32" else _abi.NR_PREC_FP64
+    fams = ("rdmesh", "chain", "logic")
+    rows, solve_s, symb_s, tune_s, conv_total = [], 0.0, 0.0, 0.0, 0
+    for fam in range(3):
+        for b in range(8):
+            cnt = int(((fam_of == fam) & (bk == b)).sum())
+            if cnt == 0:
+                continue
+            nrep = float(np.sqrt(edges[b] * edges[b + 1]))
+            if fam == 0:
+                side = max(2, int(round(np.sqrt(nrep))))
+                nl = netlists.rd_mesh(side, seed=100 + b)
+                kw = {}
+            elif fam == 1:
+                nl = netlists.inverter_chain(max(1, int(round(nrep)) - 2))
+                kw = dict(sigma_vto=0.03, sigma_kp=0.03)
+            else:
+                nl = netlists.random_logic(int(round(nrep)), seed=200 + b, p_inv=1.0)
+                kw = dict(sigma_vto=0.03, sigma_kp=0.03)
+            t0 = time.perf_counter()
+            T = nr.Topology(nl, ctx)
+            ts = time.perf_counter() - t0
+            bt = nr.Batch(ctx, T, cnt)
+            bt.set_fields(netlists.mc_fields(nl, cnt, seed=1000 + 37 * b + fam, **kw))
+            t0 = time.perf_counter()
+            if not a.no_tune:
+                bt.autotune(precision=prec, reps=2)
+            tt = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            r = bt.solve_dc(precision=prec, use_graph=1)
+            dt = time.perf_counter() - t0
+            st = bt.stats()
+            conv = int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+            conv_total += conv
+            solve_s += dt
+            symb_s += ts
+            tune_s += tt
+            rows.append({"family": fams[fam], "nodes": T.info["n"], "instances": cnt, "converged": conv,
+                         "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"], "solve_ms": dt * 1e3,
+                         "symbolic_ms": ts * 1e3, "sweeps": r["batched_iterations"],
+                         "iterations_mean": float(st["iterations"].mean()),
+                         "status_counts": np.bincount(st["status"], minlength=3).tolist()})
+            log(f"{fams[fam]} n={T.info['n']} B={cnt}: {dt * 1e3:.0f} ms, converged {conv}, symbolic {ts:.1f} s, "
+                f"tune {tt:.1f} s")
+            bt.close()
+            del bt, T
+    per_fam = {}
+    for r_ in rows:
+        x = per_fam.setdefault(r_["family"], {"instances": 0, "converged": 0, "solve_ms": 0.0})
+        x["instances"] += r_["instances"]
+        x["converged"] += r_["converged"]
+        x["solve_ms"] += r_["solve_ms"]
+    out = {
+        "metric": METRIC, "value": conv_total / solve_s, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": solve_s * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64" if prec == _abi.NR_PREC_FP32_REFINE else "f64",
+        "data": "synthetic: node counts log-uniform in [200, 50000] (seeded), families i mod 3, 8 size buckets "
+                "per family, Monte-Carlo variants per bucket",
+        "config": {"workload": f"mixed{N}: uneven-size Monte-Carlo DC operating points, three families",
+                   "instances": N, "circuits": len(rows), "precision": a.mixed_precision, "hipgraph": True,
+                   "autotuned": not a.no_tune},
+        "converged": conv_total, "symbolic_s": symb_s, "tune_s": tune_s,
+        "value_with_symbolic": conv_total / (solve_s + symb_s),
+        "families": per_fam, "circuits": rows, "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,7 +390,9 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
-    ap.add_argument("--workload", choices=["rdmesh64", "logic"], default="rdmesh64")
+    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed"], default="rdmesh64")
+    ap.add_argument("--mixed-instances", type=int, default=10000)
+    ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--logic-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--copy-probe", action="store_true")
@@ -319,6 +407,8 @@ def main():
 
     if a.workload == "logic":
         return logic_workload(a)
+    if a.workload == "mixed":
+        return mixed_workload(a)
 
     from karam_amd import _abi, netlists, nr
 

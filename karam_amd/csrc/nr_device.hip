@@ -96,6 +96,11 @@ struct DevTopo {
     const int *lin_ptr, *lin_term, *nl_ptr, *nl_term, *slot_diag;
     const int *rhs_lin_ptr, *rhs_lin_term, *rhs_nl_ptr, *rhs_nl_term;
     const int *kcl_ptr, *kcl_term;
+    const int* hub_node;             // nodes with more than KCL_HUB KCL terms
+    int n_hub;
+    // slots / rhs rows with more than HUB_TERMS device terms (k_assemble_hub)
+    const int *hub_slot, *hub_row;
+    int n_hub_slot, n_hub_row;
     const int *pn_level_ptr, *pn_diode;
     int n_pn_levels;
     const int *perm, *iperm;
@@ -278,67 +283,105 @@ __device__ __forceinline__ int block_and(int x, int* red) {
     return r;
 }
 
-// KCL residual of one node (calculatePhysicalResiduals terms,
-// physics/circuitsim.h:1250-1671, MOSFET term restated with SoA parameters)
-__device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const double* v, int i) {
+// KCL nodes with more terms than this (a supply rail feeding every PMOS
+// source) are summed by the whole block, HUB_CHUNK terms at a time
+constexpr int KCL_HUB = 256;
+constexpr int HUB_CHUNK = 1024;
+
+// One KCL term (calculatePhysicalResiduals, physics/circuitsim.h:1250-1671,
+// MOSFET term restated with SoA parameters), with its sign applied
+__device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const double* v, int t) {
 #pragma clang fp contract(off)
     const int n = T.n;
-    double s = 0.0;
-    for (int k = T.kcl_ptr[i]; k < T.kcl_ptr[i + 1]; ++k) {
-        const int t = T.kcl_term[k];
-        const int idx = term_idx(t);
-        double val;
-        switch (term_kind(t)) {
-        case karam::TK_RES:
-            val = (volt(v, n, T.res_n1[idx]) - volt(v, n, T.res_n2[idx])) / P.R[(size_t)b * T.nR + idx];
-            break;
-        case karam::TK_VSRC:
-            val = (P.vnow[(size_t)b * T.nV + idx] - (volt(v, n, T.vsrc_pos[idx]) - volt(v, n, T.vsrc_neg[idx]))) /
-                  1e-3;
-            break;
-        case karam::TK_CAP: {   // backward-Euler displacement current
-            const double vc = volt(v, n, T.cap_n1[idx]) - volt(v, n, T.cap_n2[idx]);
-            val = (P.cap[(size_t)b * T.nC + idx] / P.h) * (vc - P.cap_vprev[(size_t)b * T.nC + idx]);
-            break;
-        }
-        case karam::TK_ISRC:
-            val = P.isrc[(size_t)b * T.nI + idx];
-            break;
-        case karam::TK_DIODE: {
-            double vd = volt(v, n, T.d_a[idx]) - volt(v, n, T.d_c[idx]);
-            val = diode_i_clamped(vd, P.Is[(size_t)b * T.nD + idx], T.d_n[idx], T.d_vtraw[idx]);
-            break;
-        }
-        default: {  // TK_MOS
-            double vgs, vds, ids, gm, gds;
-            mos_eval(volt(v, n, T.m_g[idx]), volt(v, n, T.m_d[idx]), volt(v, n, T.m_s[idx]), T.m_w[idx],
-                     T.m_l[idx], P.kp[(size_t)b * T.nM + idx], P.vth[(size_t)b * T.nM + idx], T.m_lam[idx],
-                     T.m_pmos[idx], vgs, vds, ids, gm, gds);
-            val = ids;
-            break;
-        }
-        }
-        s = term_neg(t) ? s + (-val) : s + val;
+    const int idx = term_idx(t);
+    double val;
+    switch (term_kind(t)) {
+    case karam::TK_RES:
+        val = (volt(v, n, T.res_n1[idx]) - volt(v, n, T.res_n2[idx])) / P.R[(size_t)b * T.nR + idx];
+        break;
+    case karam::TK_VSRC:
+        val = (P.vnow[(size_t)b * T.nV + idx] - (volt(v, n, T.vsrc_pos[idx]) - volt(v, n, T.vsrc_neg[idx]))) /
+              1e-3;
+        break;
+    case karam::TK_CAP: {   // backward-Euler displacement current
+        const double vc = volt(v, n, T.cap_n1[idx]) - volt(v, n, T.cap_n2[idx]);
+        val = (P.cap[(size_t)b * T.nC + idx] / P.h) * (vc - P.cap_vprev[(size_t)b * T.nC + idx]);
+        break;
     }
+    case karam::TK_ISRC:
+        val = P.isrc[(size_t)b * T.nI + idx];
+        break;
+    case karam::TK_DIODE: {
+        double vd = volt(v, n, T.d_a[idx]) - volt(v, n, T.d_c[idx]);
+        val = diode_i_clamped(vd, P.Is[(size_t)b * T.nD + idx], T.d_n[idx], T.d_vtraw[idx]);
+        break;
+    }
+    default: {  // TK_MOS
+        double vgs, vds, ids, gm, gds;
+        mos_eval(volt(v, n, T.m_g[idx]), volt(v, n, T.m_d[idx]), volt(v, n, T.m_s[idx]), T.m_w[idx],
+                 T.m_l[idx], P.kp[(size_t)b * T.nM + idx], P.vth[(size_t)b * T.nM + idx], T.m_lam[idx],
+                 T.m_pmos[idx], vgs, vds, ids, gm, gds);
+        val = ids;
+        break;
+    }
+    }
+    return term_neg(t) ? -val : val;
+}
+
+// KCL residual of one node: its terms summed in list order
+__device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const double* v, int i) {
+#pragma clang fp contract(off)
+    double s = 0.0;
+    for (int k = T.kcl_ptr[i]; k < T.kcl_ptr[i + 1]; ++k) s = s + kcl_term_val(T, P, b, v, T.kcl_term[k]);
     return s;
+}
+
+// max |KCL| over the nodes of instance b (whole block); put(i, r) receives
+// every node's value.  A hub node's terms are evaluated by all threads into
+// LDS and added by thread 0 in list order, so each node's value is the same
+// sequential sum as kcl_node's; the maximum does not depend on the order.
+template <typename Put>
+__device__ double block_kcl_put(const DevTopo& T, const DevBatch& P, int b, const double* v, double* red, Put put) {
+#pragma clang fp contract(off)
+    __shared__ double hbuf[HUB_CHUNK];
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) {
+        if (T.kcl_ptr[i + 1] - T.kcl_ptr[i] > KCL_HUB) continue;
+        const double r = kcl_node(T, P, b, v, i);
+        put(i, r);
+        mx = ref_max(mx, fabs(r));
+    }
+    for (int h = 0; h < T.n_hub; ++h) {
+        const int i = T.hub_node[h];
+        const int k0 = T.kcl_ptr[i], k1 = T.kcl_ptr[i + 1];
+        double s = 0.0;
+        for (int base = k0; base < k1; base += HUB_CHUNK) {
+            const int cnt = min(HUB_CHUNK, k1 - base);
+            __syncthreads();
+            for (int q = threadIdx.x; q < cnt; q += blockDim.x) hbuf[q] = kcl_term_val(T, P, b, v, T.kcl_term[base + q]);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int q = 0; q < cnt; ++q) s = s + hbuf[q];
+        }
+        if (threadIdx.x == 0) {
+            put(i, s);
+            mx = ref_max(mx, fabs(s));
+        }
+    }
+    return block_max(mx, red);
 }
 
 // max |KCL| over the nodes of instance b; `out` (optional) receives the
 // per-node values
 __device__ double block_kcl(const DevTopo& T, const DevBatch& P, int b, const double* v, double* red,
                             double* out = nullptr) {
-    double mx = 0.0;
-    for (int i = threadIdx.x; i < T.n; i += blockDim.x) {
-        const double r = kcl_node(T, P, b, v, i);
-        if (out) out[i] = r;
-        mx = ref_max(mx, fabs(r));
-    }
-    return block_max(mx, red);
+    return block_kcl_put(T, P, b, v, red, [&](int i, double r) { if (out) out[i] = r; });
 }
 
-// FP32 path: the interleaved KCL vector at v (one block per instance)
+// FP32 path: the interleaved KCL vector at v (whole block per instance)
 __device__ void residual_to_interleaved(const DevTopo& T, const DevBatch& P, int b, const double* v) {
-    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = kcl_node(T, P, b, v, i);
+    __shared__ double red[8];
+    (void)block_kcl_put(T, P, b, v, red, [&](int i, double r) { P.rhslin[IX(i, b, P.Bp)] = r; });
 }
 
 // linear rhs of node i in emission order: Norton V sources (at the values in
@@ -551,7 +594,9 @@ __global__ __launch_bounds__(64) void k_wave_start(DevTopo T, DevBatch P, int st
 
 __global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
 #pragma clang fp contract(off)
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     LANE_SETUP(4)
     const int n = T.n;
     auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
@@ -576,25 +621,68 @@ __global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* 
     }
 }
 
+// Slots (and rhs rows) with more device terms than this are assembled by
+// k_assemble_hub, which keeps k_assemble's register use low
+constexpr int HUB_TERMS = 16;
+
+// Sum of a term list in list order, starting from `acc`.  Short lists (the
+// usual one or two device stamps of a slot) go term by term; long ones (a
+// supply rail's diagonal holds one term per device) are loaded 16 at a time
+// with indices clamped to the last term, so the loads of a batch are all in
+// flight together.  load(t) returns the value of term t, add(acc, t, x) adds it.
+template <typename A, typename Load, typename Add>
+__device__ __forceinline__ A sum_terms(A acc, const int* __restrict__ terms, int k0, int k1, Load load, Add add) {
+    if (k1 - k0 <= 8) {
+        for (int k = k0; k < k1; ++k) {
+            const int t = terms[k];
+            acc = add(acc, t, load(t));
+        }
+        return acc;
+    }
+    int kb = k0;
+    for (; k1 - kb > 4; kb += 16) {
+        int tt[16];
+        A x[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) tt[u] = terms[min(kb + u, k1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) x[u] = load(tt[u]);
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (kb + u < k1) acc = add(acc, tt[u], x[u]);
+    }
+    if (kb < k1) {
+        int tt[4];
+        A x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) tt[u] = terms[min(kb + u, k1 - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = load(tt[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (kb + u < k1) acc = add(acc, tt[u], x[u]);
+    }
+    return acc;
+}
+
 // A = linear part + device stamps (emission order) + gmin; rhs likewise.
 // Each value goes to the front that owns it: the interleaved small-front pool
 // or the instance-major large-front arrays.
 __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                   const double* __restrict__ gmin_tab, int gmin_last) {
 #pragma clang fp contract(off)
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     LANE_SETUP(4)
     if (item < T.nnz_a) {
+        if (T.nl_ptr[item + 1] - T.nl_ptr[item] > HUB_TERMS) goto rhs;   // k_assemble_hub
         double a = P.glin[IX(item, b, Bp)];
         for (int k = T.nl_ptr[item]; k < T.nl_ptr[item + 1]; ++k) {
             const int t = T.nl_term[k];
-            const int idx = term_idx(t);
-            double g;
-            switch (term_kind(t)) {
-            case karam::SK_DIODE: g = P.dg[IX(idx, b, Bp)]; break;
-            case karam::SK_MOS_GDS: g = P.mgds[IX(idx, b, Bp)]; break;
-            default: g = P.mgm[IX(idx, b, Bp)]; break;
-            }
+            const int kind = term_kind(t);
+            const double* src = kind == karam::SK_DIODE ? P.dg : kind == karam::SK_MOS_GDS ? P.mgds : P.mgm;
+            const double g = src[IX(term_idx(t), b, Bp)];
             a = term_neg(t) ? a + (-g) : a + g;
         }
         if (T.slot_diag[item]) a = a + gmin_tab[min(P.st[b].step, gmin_last)];
@@ -602,13 +690,14 @@ __global__ __launch_bounds__(256) void k_assemble(DevTopo T, DevBatch P, const i
         if (d & 1) P.Al[IX(d >> 1, b, Bp)] = a;
         else P.pool[IX(d >> 1, b, Bp)] = a;
     }
+rhs:
     if (item < T.n) {
+        if (T.rhs_nl_ptr[item + 1] - T.rhs_nl_ptr[item] > HUB_TERMS) return;   // k_assemble_hub
         double r = P.rhslin[IX(item, b, Bp)];
         for (int k = T.rhs_nl_ptr[item]; k < T.rhs_nl_ptr[item + 1]; ++k) {
             const int t = T.rhs_nl_term[k];
-            const int idx = term_idx(t);
-            double val = (term_kind(t) == karam::TK_DIODE) ? P.dieq[IX(idx, b, Bp)] : P.mieq[IX(idx, b, Bp)];
-            r = term_neg(t) ? r + (-val) : r + val;
+            const double x = (term_kind(t) == karam::TK_DIODE ? P.dieq : P.mieq)[IX(term_idx(t), b, Bp)];
+            r = term_neg(t) ? r + (-x) : r + x;
         }
         const int d = T.node_dst[item];
         if (d & 1) P.rhsl[IX(d >> 1, b, Bp)] = r;
@@ -650,7 +739,9 @@ __device__ __forceinline__ void mos_g32(double vgs64, double vds64, double W, do
 
 __global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
 #pragma clang fp contract(off)
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     LANE_SETUP(4)
     const int n = T.n;
     auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
@@ -675,7 +766,9 @@ __global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int
 __global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                     const double* __restrict__ gmin_tab, int gmin_last) {
 #pragma clang fp contract(off)
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     LANE_SETUP(4)
     float* pool = reinterpret_cast<float*>(P.pool);
     const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
@@ -683,16 +776,12 @@ __global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const
         const float* dg = reinterpret_cast<const float*>(P.dg);
         const float* mgds = reinterpret_cast<const float*>(P.mgds);
         const float* mgm = reinterpret_cast<const float*>(P.mgm);
+        if (T.nl_ptr[item + 1] - T.nl_ptr[item] > HUB_TERMS) goto rhs;   // k_assemble_hub
         float a = (float)P.glin[IX(item, b, Bp)];
         for (int k = T.nl_ptr[item]; k < T.nl_ptr[item + 1]; ++k) {
             const int t = T.nl_term[k];
-            const int idx = term_idx(t);
-            float g;
-            switch (term_kind(t)) {
-            case karam::SK_DIODE: g = dg[IX(idx, b, Bp)]; break;
-            case karam::SK_MOS_GDS: g = mgds[IX(idx, b, Bp)]; break;
-            default: g = mgm[IX(idx, b, Bp)]; break;
-            }
+            const int kind = term_kind(t);
+            const float g = (kind == karam::SK_DIODE ? dg : kind == karam::SK_MOS_GDS ? mgds : mgm)[IX(term_idx(t), b, Bp)];
             a = term_neg(t) ? a + (-g) : a + g;
         }
         if (T.slot_diag[item]) a = a + (float)gmin;
@@ -700,11 +789,63 @@ __global__ __launch_bounds__(256) void k_assemble32(DevTopo T, DevBatch P, const
         if (d & 1) reinterpret_cast<float*>(P.Al)[IX(d >> 1, b, Bp)] = a;
         else pool[IX(d >> 1, b, Bp)] = a;
     }
+rhs:
     if (item < T.n) {
         const float r = (float)(P.rhslin[IX(item, b, Bp)] - gmin * P.vI[IX(item, b, Bp)]);
         const int d = T.node_dst[item];
         if (d & 1) reinterpret_cast<float*>(P.rhsl)[IX(d >> 1, b, Bp)] = r;
         else pool[IX(d >> 1, b, Bp)] = r;
+    }
+}
+
+// The A entries and FP64 rhs rows with more than HUB_TERMS device terms
+// (a supply rail), one thread per (item, instance): the same sums in the
+// same order as k_assemble / k_assemble32, with the loads batched.
+__global__ __launch_bounds__(256) void k_assemble_hub(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                      int nact_grid, const double* __restrict__ gmin_tab,
+                                                      int gmin_last) {
+#pragma clang fp contract(off)
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    LANE_SETUP(4)
+    const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
+    if (item < T.n_hub_slot) {
+        const int sl = T.hub_slot[item];
+        const int d = T.slot_dst[sl];
+        if (P.p32) {
+            const float* dg = reinterpret_cast<const float*>(P.dg);
+            const float* mgds = reinterpret_cast<const float*>(P.mgds);
+            const float* mgm = reinterpret_cast<const float*>(P.mgm);
+            float a = sum_terms(
+                (float)P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1],
+                [&](int t) {
+                    const int kind = term_kind(t);
+                    return (kind == karam::SK_DIODE ? dg : kind == karam::SK_MOS_GDS ? mgds : mgm)[IX(term_idx(t), b, Bp)];
+                },
+                [](float acc, int t, float g) { return term_neg(t) ? acc + (-g) : acc + g; });
+            if (T.slot_diag[sl]) a = a + (float)gmin;
+            if (d & 1) reinterpret_cast<float*>(P.Al)[IX(d >> 1, b, Bp)] = a;
+            else reinterpret_cast<float*>(P.pool)[IX(d >> 1, b, Bp)] = a;
+        } else {
+            double a = sum_terms(
+                P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1],
+                [&](int t) {
+                    const int kind = term_kind(t);
+                    return (kind == karam::SK_DIODE ? P.dg : kind == karam::SK_MOS_GDS ? P.mgds : P.mgm)[IX(term_idx(t), b, Bp)];
+                },
+                [](double acc, int t, double g) { return term_neg(t) ? acc + (-g) : acc + g; });
+            if (T.slot_diag[sl]) a = a + gmin;
+            if (d & 1) P.Al[IX(d >> 1, b, Bp)] = a;
+            else P.pool[IX(d >> 1, b, Bp)] = a;
+        }
+    } else if (item < T.n_hub_slot + T.n_hub_row && !P.p32) {
+        const int i = T.hub_row[item - T.n_hub_slot];
+        const double r = sum_terms(
+            P.rhslin[IX(i, b, Bp)], T.rhs_nl_term, T.rhs_nl_ptr[i], T.rhs_nl_ptr[i + 1],
+            [&](int t) { return (term_kind(t) == karam::TK_DIODE ? P.dieq : P.mieq)[IX(term_idx(t), b, Bp)]; },
+            [](double acc, int t, double x) { return term_neg(t) ? acc + (-x) : acc + x; });
+        const int d = T.node_dst[i];
+        if (d & 1) P.rhsl[IX(d >> 1, b, Bp)] = r;
+        else P.pool[IX(d >> 1, b, Bp)] = r;
     }
 }
 
@@ -835,7 +976,9 @@ template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                       const int* __restrict__ sub_ptr,
                                                       const int* __restrict__ sub_sn, int n_sub) {
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     const int lane = threadIdx.x & 63;
     const long long task = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int ngroups = (nact + 63) / 64;
@@ -853,7 +996,9 @@ template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                          int nact_grid, const int* __restrict__ sub_ptr,
                                                          const int* __restrict__ sub_sn, int n_sub) {
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     const int lane = threadIdx.x & 63;
     const long long task = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int ngroups = (nact + 63) / 64;
@@ -903,7 +1048,9 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     // a team wider than a wavefront synchronises the whole block, so it must
     // be alone in it (early exits and per-front barrier counts then agree)
     static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
@@ -1104,7 +1251,9 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     // a team wider than a wavefront synchronises the whole block, so it must
     // be alone in it (early exits and per-front barrier counts then agree)
     static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     V* smem = reinterpret_cast<V*>(smem_raw);
     constexpr int TPB = NT / TEAM;
@@ -1150,7 +1299,9 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
 // xs (interleaved, permuted rows) -> xm (instance-major, permuted rows, FP64)
 template <typename V>
 __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64, g = blockIdx.y;
@@ -1174,7 +1325,9 @@ __global__ __launch_bounds__(256) void k_x_to_xm(DevTopo T, DevBatch P, const in
 // v (instance-major) -> vI (interleaved); on the FP32 path also the KCL
 // vector res -> rhslin
 __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
-    const int nact = *P.nact;   // live active count (<= the grid's nact_grid)
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
     __shared__ double tile[64][65];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int row0 = blockIdx.x * 64, g = blockIdx.y;

@@ -46,6 +46,13 @@ enum {
 
 /* ---- netlist (caller-owned, netlist order) ------------------------------- */
 
+/* Transient waveforms of independent sources (VoltageSource / CurrentSource
+ * type, netlist/circuit.h:143-171; evaluated as stampSoABlock does,
+ * physics/circuitsim.h:2150-2240).  A DC solve uses the DC value.
+ *   NR_WAVE_PULSE: v1 v2 td tr tf pw per
+ *   NR_WAVE_SINE:  vo va freq td theta phase_deg (7th value unused)      */
+enum { NR_WAVE_DC = 0, NR_WAVE_PULSE = 1, NR_WAVE_SINE = 2 };
+
 /* A `.model` card (netlist/circuit.h:31-41).  `type` is "NMOS", "PMOS" or
  * "D"; has_* flags say whether the card sets the parameter (ModelCard::get
  * falls back to the tensorizer defaults otherwise). */
@@ -74,13 +81,15 @@ typedef struct nr_netlist {
     const int32_t* vsrc_pos;
     const int32_t* vsrc_neg;
     const double* vsrc_volts;   /* DC value (voltage_V)                      */
-    const int32_t* vsrc_is_pulse;  /* nullable: 1 = PULSE waveform           */
-    const double* vsrc_pulse;   /* nullable: 7 per source v1 v2 td tr tf pw per */
+    const int32_t* vsrc_wave;   /* nullable: NR_WAVE_* per source            */
+    const double* vsrc_wave_par; /* nullable: 7 per source, see NR_WAVE_*    */
 
     int32_t n_isrc;             /* CurrentSource                             */
     const int32_t* isrc_pos;
     const int32_t* isrc_neg;
-    const double* isrc_amps;
+    const double* isrc_amps;    /* DC value (current_A)                      */
+    const int32_t* isrc_wave;   /* nullable: NR_WAVE_* per source            */
+    const double* isrc_wave_par; /* nullable: 7 per source, see NR_WAVE_*    */
 
     int32_t n_diode;            /* Diode (circuit.h:392-408)                 */
     const int32_t* dio_anode;
@@ -237,7 +246,7 @@ typedef struct nr_result {
 NR_API int nr_batch_solve_dc(nr_batch* batch, const nr_solve_opts* opts,
                              nr_result* result);
 
-/* ---- transient (backward Euler, fixed step) ----------------------------- */
+/* ---- transient ------------------------------------------------------------ */
 /* A transient topology also holds the capacitor companion stamps in its
  * pattern, as stepTransient stamps them (solvers/circuitsim_solver.cpp:
  * 559-780, physics/circuitsim.h:2479-2505).  nr_topology_build builds the
@@ -247,20 +256,46 @@ NR_API int nr_topology_build_ex(nr_ctx* ctx, const nr_netlist* nl, int32_t kind,
                                 nr_topology** out);
 
 enum {
-    NR_TRAN_START_ZERO = 0,  /* zero voltages and capacitor history (the
+    NR_TRAN_START_ZERO = 0,  /* zero voltages, history and arbiter state (the
                                 reference's start, D11)                     */
-    NR_TRAN_START_SEED = 1   /* the batch seed, e.g. a DC operating point   */
+    NR_TRAN_START_SEED = 1   /* the batch seed, e.g. a DC operating point:
+                                capacitor history at its branch voltages
+                                with zero current, arbiter at that state    */
+};
+/* Capacitor companion models (IntegrationType, math/integration_method.h:
+ * 15-19, same numbering). */
+enum { NR_INT_TRAP = 0, NR_INT_GEAR1 = 1, NR_INT_GEAR2 = 2 };
+/* Step control. */
+enum {
+    NR_TRAN_FIXED = 0,       /* `steps` steps of h; a step whose Newton solve
+                                fails ends the instance                     */
+    NR_TRAN_ARBITER = 1      /* stepTransient + its caller (solvers/
+                                circuitsim_solver.cpp:559-780, engine_impl.cpp:
+                                205-256): every step starts at h, at most 4
+                                attempts halving h on a failed Newton solve or
+                                an LTE / energy rejection by the
+                                IntegratorArbiter (physics/circuitsim.h:57-368),
+                                trapezoidal switched to Gear-2 after 3
+                                rejections in a row, until t >= t_stop      */
 };
 typedef struct nr_transient_opts {
-    double h;                /* fixed time step (s)                          */
-    int32_t steps;           /* steps to take                                */
-    int32_t max_iter;        /* NR iterations per step, MAX_NR_ITER = 50     */
+    double h;                /* time step (s); the arbiter's requested step  */
+    int32_t steps;           /* NR_TRAN_FIXED: steps to take                 */
+    int32_t max_iter;        /* NR iterations per attempt, MAX_NR_ITER = 50  */
     int32_t start;           /* NR_TRAN_START_*                              */
-    int32_t n_probe;         /* nodes recorded at every step                 */
+    int32_t n_probe;         /* nodes recorded in the waveform ring          */
     const int32_t* probe;    /* 1-based node ids, n_probe of them            */
     int32_t record_timing;   /* as in nr_solve_opts                          */
     int32_t precision;       /* NR_PREC_*, as in nr_solve_opts               */
     int32_t use_graph;       /* as in nr_solve_opts                          */
+    int32_t method;          /* NR_INT_*: the companion model at t = 0       */
+    int32_t control;         /* NR_TRAN_*                                    */
+    double t_stop;           /* NR_TRAN_ARBITER: stop time (s), <= 1e8 h     */
+    int32_t wave_capacity;   /* waveform ring entries per instance; 0: steps+1
+                                (fixed) or ceil(t_stop / h) + 1 (arbiter)   */
+    int32_t wave_decimation; /* record every k-th accepted step (>= 1), as
+                                recordWaveform (shaders/gpu_nr_loop.wgsl:
+                                862-891)                                    */
 } nr_transient_opts;
 NR_API void nr_transient_opts_default(nr_transient_opts* opts);
 
@@ -288,18 +323,44 @@ typedef struct nr_autotune_result {
 NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
                              nr_autotune_result* out);
 
-/* Fixed-step backward-Euler transient of every instance (the restated
- * stepTransient: companion G = C/h with history (C/h) v_prev, PULSE sources
- * at t_{n+1}, Newton from the previous step's solution with the 1e-12
- * diagonal and no limiting or damping, converged by checkConvergence with
- * the capacitor current in the KCL audit).  The batch's topology must be of
- * kind NR_TOPO_TRANSIENT.  In nr_instance_stats, iterations count Newton
- * iterations over all steps and gmin_steps_done counts completed steps. */
+/* Transient of every instance (the restated stepTransient): capacitor
+ * companions of the instance's current method (trapezoidal G = 2C/h with
+ * history current, Gear-1 G = C/h, Gear-2 G = 1.5C/h with two voltages of
+ * history; math/integration_method.h:47-157), PULSE / SINE sources at
+ * t + h, Newton from the arbiter's last voltages (the last accepted step)
+ * with the 1e-12 diagonal and no limiting or damping, converged by
+ * checkConvergence with the method's capacitor current in the KCL audit,
+ * history shifted on acceptance (solvers/circuitsim_solver.cpp:668-699).
+ * Instances advance independently: each has its own time, step, method and
+ * arbiter state.  The batch's topology must be of kind NR_TOPO_TRANSIENT.
+ * In nr_instance_stats, iterations count Newton iterations over all
+ * attempts and gmin_steps_done counts accepted steps. */
 NR_API int nr_batch_transient(nr_batch* batch, const nr_transient_opts* opts,
                               nr_result* result);
-/* Probe voltages of the last transient: out[b][s][p], s = 0..steps (s = 0 is
- * the start state); steps an instance did not complete hold NaN. */
+/* Waveform ring of the last transient (per instance, `capacity` entries of a
+ * time and n_probe voltages).  Entry k (k = 0 is the start state at t = 0,
+ * then every wave_decimation-th accepted step) is stored in slot
+ * k mod capacity; slots never written hold NaN.
+ *   nr_batch_get_waveform:       out[b][slot][p], count = B*capacity*n_probe
+ *   nr_batch_get_waveform_times: out[b][slot],    count = B*capacity
+ *   nr_batch_get_wave_recorded:  out[b] = entries written, count = B      */
 NR_API int nr_batch_get_waveform(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_waveform_times(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_wave_recorded(const nr_batch* batch, int64_t* out, int64_t count);
+/* Per-instance step control of the last transient. */
+typedef struct nr_tran_stats {
+    double t_end;            /* time of the last accepted step (s)           */
+    double h_last;           /* step of the last attempt (s)                 */
+    int32_t accepted;        /* accepted steps                               */
+    int32_t rejected;        /* attempts not accepted (Newton failure, LTE /
+                                energy rejection, method switch)            */
+    int32_t method;          /* NR_INT_* at the end                          */
+    int32_t switched_at;     /* accepted steps before the switch to Gear-2, -1
+                                if none                                     */
+    int32_t instability;     /* the arbiter's instability counter at the end */
+    int32_t pad;
+} nr_tran_stats;
+NR_API int nr_batch_get_tran_stats(const nr_batch* batch, nr_tran_stats* out, int64_t count);
 
 /* Per-instance results.  out arrays are caller-owned host buffers. */
 NR_API int nr_batch_get_voltages(const nr_batch* batch, double* out, int64_t count);

@@ -50,6 +50,14 @@ NR_TOPO_DC = 0
 NR_TOPO_TRANSIENT = 1
 NR_TRAN_START_ZERO = 0
 NR_TRAN_START_SEED = 1
+NR_WAVE_DC = 0
+NR_WAVE_PULSE = 1
+NR_WAVE_SINE = 2
+NR_INT_TRAP = 0
+NR_INT_GEAR1 = 1
+NR_INT_GEAR2 = 2
+NR_TRAN_FIXED = 0
+NR_TRAN_ARBITER = 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -69,8 +77,9 @@ class NrNetlist(C.Structure):
         ("n_res", C.c_int32), ("res_n1", _i32p), ("res_n2", _i32p), ("res_ohms", _f64p),
         ("n_cap", C.c_int32), ("cap_n1", _i32p), ("cap_n2", _i32p), ("cap_farads", _f64p),
         ("n_vsrc", C.c_int32), ("vsrc_pos", _i32p), ("vsrc_neg", _i32p), ("vsrc_volts", _f64p),
-        ("vsrc_is_pulse", _i32p), ("vsrc_pulse", _f64p),
+        ("vsrc_wave", _i32p), ("vsrc_wave_par", _f64p),
         ("n_isrc", C.c_int32), ("isrc_pos", _i32p), ("isrc_neg", _i32p), ("isrc_amps", _f64p),
+        ("isrc_wave", _i32p), ("isrc_wave_par", _f64p),
         ("n_diode", C.c_int32), ("dio_anode", _i32p), ("dio_cathode", _i32p),
         ("dio_is", _f64p), ("dio_n", _f64p), ("dio_vt", _f64p), ("dio_model", _i32p),
         ("n_mos", C.c_int32), ("mos_d", _i32p), ("mos_g", _i32p), ("mos_s", _i32p), ("mos_b", _i32p),
@@ -106,7 +115,15 @@ class NrTransientOpts(C.Structure):
     _fields_ = [
         ("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
         ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)), ("record_timing", C.c_int32),
-        ("precision", C.c_int32), ("use_graph", C.c_int32),
+        ("precision", C.c_int32), ("use_graph", C.c_int32), ("method", C.c_int32), ("control", C.c_int32),
+        ("t_stop", C.c_double), ("wave_capacity", C.c_int32), ("wave_decimation", C.c_int32),
+    ]
+
+
+class NrTranStats(C.Structure):
+    _fields_ = [
+        ("t_end", C.c_double), ("h_last", C.c_double), ("accepted", C.c_int32), ("rejected", C.c_int32),
+        ("method", C.c_int32), ("switched_at", C.c_int32), ("instability", C.c_int32), ("pad", C.c_int32),
     ]
 
 

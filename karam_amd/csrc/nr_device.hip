@@ -119,10 +119,10 @@ struct DevTopo {
     // fronts whose panel is kept in HBM: offset of their panel (front-major)
     const long long* sn_gp_off;
     long long zero_item;
-    // transient: capacitors and PULSE sources (v1 v2 td tr tf pw per)
+    // transient: capacitors and source waveforms (NR_WAVE_*, 7 values each)
     int nC;
-    const int *cap_n1, *cap_n2, *vsrc_is_pulse;
-    const double* vsrc_pulse;
+    const int *cap_n1, *cap_n2, *vsrc_wave, *isrc_wave;
+    const double *vsrc_wave_par, *isrc_wave_par;
 };
 
 struct InstState {
@@ -130,6 +130,19 @@ struct InstState {
     int status, done, singular, seeded;
     double prev_kcl, kcl;
 };
+
+// Per-instance transient step control: the instance's time, the step and
+// companion method of the current attempt, and the IntegratorArbiter's
+// scalars (physics/circuitsim.h:57-66).  glin_h / glin_method: the step
+// and method the linear part of A was last assembled for.
+struct TranState {
+    double t, h, prev_energy, prev_dt, glin_h;
+    long long rec;               // waveform entries written
+    int method, attempt, instab, accepted, rejected, switched_at, glin_method, dec;
+};
+// InstState.done in a transient: 0 solving, 1 finished, 2 the attempt's
+// Newton solve converged, 3 it failed (the step control decides)
+enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3 };
 
 struct DevBatch {
     int B, Bp;
@@ -149,12 +162,16 @@ struct DevBatch {
     double *Al, *rhsl, *UL, *CBL, *RCBL;
     double* gpanel;              // HBM panels of the fronts too large for LDS
     double* seed;                // optional [B][n]
-    // transient: capacitances, capacitor voltages of the last accepted step,
-    // source values at the current time, the step h; vnow = the source values
-    // the KCL audit uses (vsrc for DC, vsrc_t in a transient)
-    double *cap, *cap_vprev, *vsrc_t;
-    const double* vnow;
-    double h;
+    // transient: capacitances, capacitor history (branch voltages of the
+    // last two accepted steps, current of the last), source values at the
+    // attempt's time; vnow / inow = the source values the stamps and the KCL
+    // audit use (vsrc / isrc for DC, vsrc_t / isrc_t in a transient); the
+    // arbiter's last two stable solutions (instance-major [B][n]); ts = the
+    // per-instance step control (NULL for DC)
+    double *cap, *hv0, *hv1, *hi0, *vsrc_t, *isrc_t;
+    const double *vnow, *inow;
+    double *arb_prev, *arb_prev2;
+    TranState* ts;
     // FP32 path (p32 = 1): the FP64 KCL vector of the current iterate
     // (instance-major); its interleaved copy lives in rhslin, which the FP32
     // path does not otherwise use
@@ -303,13 +320,18 @@ __device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const
         val = (P.vnow[(size_t)b * T.nV + idx] - (volt(v, n, T.vsrc_pos[idx]) - volt(v, n, T.vsrc_neg[idx]))) /
               1e-3;
         break;
-    case karam::TK_CAP: {   // backward-Euler displacement current
+    case karam::TK_CAP: {   // displacement current of the attempt's companion (:1460-1490)
+        const size_t ci = (size_t)b * T.nC + idx;
         const double vc = volt(v, n, T.cap_n1[idx]) - volt(v, n, T.cap_n2[idx]);
-        val = (P.cap[(size_t)b * T.nC + idx] / P.h) * (vc - P.cap_vprev[(size_t)b * T.nC + idx]);
+        const double C = P.cap[ci], h = P.ts[b].h;
+        const int m = P.ts[b].method;
+        if (m == NR_INT_TRAP) val = ((2.0 * C) / h) * (vc - P.hv0[ci]) - P.hi0[ci];
+        else if (m == NR_INT_GEAR2) val = (C / h) * (1.5 * vc - 2.0 * P.hv0[ci] + 0.5 * P.hv1[ci]);
+        else val = (C / h) * (vc - P.hv0[ci]);
         break;
     }
     case karam::TK_ISRC:
-        val = P.isrc[(size_t)b * T.nI + idx];
+        val = P.inow[(size_t)b * T.nI + idx];
         break;
     case karam::TK_DIODE: {
         double vd = volt(v, n, T.d_a[idx]) - volt(v, n, T.d_c[idx]);
@@ -384,8 +406,36 @@ __device__ void residual_to_interleaved(const DevTopo& T, const DevBatch& P, int
     (void)block_kcl_put(T, P, b, v, red, [&](int i, double r) { P.rhslin[IX(i, b, P.Bp)] = r; });
 }
 
+// Capacitor companion of the attempt's method (math/integration_method.h:
+// 47-157): its conductance, and the history current it injects into its
+// first plate (trapezoidal -I_eq = G v0 + i0, Gear-1 G v0, Gear-2
+// C (2/h v0 - 0.5/h v1)), each written as the reference computes it
+__device__ double cap_conductance(const DevBatch& P, int b, int nC, int idx) {
+#pragma clang fp contract(off)
+    const double C = P.cap[(size_t)b * nC + idx], h = P.ts[b].h;
+    const int m = P.ts[b].method;
+    return m == NR_INT_TRAP ? (2.0 * C) / h : m == NR_INT_GEAR2 ? (1.5 * C) / h : C / h;
+}
+
+__device__ double cap_history_current(const DevBatch& P, int b, int nC, int idx) {
+#pragma clang fp contract(off)
+    const size_t ci = (size_t)b * nC + idx;
+    const double C = P.cap[ci], h = P.ts[b].h;
+    switch (P.ts[b].method) {
+    case NR_INT_TRAP: {
+        const double G = (2.0 * C) / h;
+        return -(-G * P.hv0[ci] - P.hi0[ci]);
+    }
+    case NR_INT_GEAR2: return C * ((2.0 / h) * P.hv0[ci] - (0.5 / h) * P.hv1[ci]);
+    default: {
+        const double G = C / h;
+        return -(-G * P.hv0[ci]);
+    }
+    }
+}
+
 // linear rhs of node i in emission order: Norton V sources (at the values in
-// P.vnow), I sources, then the capacitor history currents (C/h) v_prev
+// P.vnow), I sources (P.inow), then the capacitor history currents
 __device__ double rhs_lin_node(const DevTopo& T, const DevBatch& P, int b, int i) {
 #pragma clang fp contract(off)
     double r = 0.0;
@@ -395,10 +445,8 @@ __device__ double rhs_lin_node(const DevTopo& T, const DevBatch& P, int b, int i
         double val;
         switch (term_kind(t)) {
         case karam::TK_VSRC: val = P.vnow[(size_t)b * T.nV + idx] * 1.0e3; break;
-        case karam::TK_CAP:
-            val = (P.cap[(size_t)b * T.nC + idx] / P.h) * P.cap_vprev[(size_t)b * T.nC + idx];
-            break;
-        default: val = P.isrc[(size_t)b * T.nI + idx]; break;
+        case karam::TK_CAP: val = cap_history_current(P, b, T.nC, idx); break;
+        default: val = P.inow[(size_t)b * T.nI + idx]; break;
         }
         r = term_neg(t) ? r + (-val) : r + val;
     }
@@ -418,6 +466,84 @@ __device__ double pulse_value(const double* p, double time) {
     if (t < tr + pw) return v2;
     if (t < tr + pw + tf) return v2 + (v1 - v2) * ((t - tr - pw) / tf);
     return v1;
+}
+
+// SINE source value at time t (physics/circuitsim.h:2171-2180);
+// p = vo va freq td theta phase_deg
+__device__ double sine_value(const double* p, double time) {
+#pragma clang fp contract(off)
+    const double vo = p[0], va = p[1], freq = p[2], td = p[3], theta = p[4], ph = p[5];
+    const double t = time - td;
+    if (t > 0) {
+        const double omega = 2.0 * 3.141592653589793 * freq;
+        const double damping = exp(-t * theta);
+        const double phase = ph * (3.141592653589793 / 180.0);
+        return vo + va * sin(omega * t + phase) * damping;
+    }
+    return vo;
+}
+
+__device__ double wave_value(int kind, const double* par, double dc, double time) {
+    return kind == NR_WAVE_PULSE ? pulse_value(par, time) : kind == NR_WAVE_SINE ? sine_value(par, time) : dc;
+}
+
+// Sum of f(0..count-1) added in index order from 0.0 (the reference's
+// sequential accumulation), by the whole block: terms are evaluated in
+// parallel into LDS, thread 0 adds them; every thread gets the sum.
+template <typename F>
+__device__ double block_ordered_sum(int count, F f) {
+#pragma clang fp contract(off)
+    __shared__ double buf[HUB_CHUNK];
+    __shared__ double out;
+    double s = 0.0;
+    for (int base = 0; base < count; base += HUB_CHUNK) {
+        const int cnt = min(HUB_CHUNK, count - base);
+        __syncthreads();
+        for (int q = threadIdx.x; q < cnt; q += blockDim.x) buf[q] = f(base + q);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int q = 0; q < cnt; ++q) s = s + buf[q];
+    }
+    if (threadIdx.x == 0) out = s;
+    __syncthreads();
+    const double r = out;
+    __syncthreads();
+    return r;
+}
+
+// stored capacitor energy sum 0.5 C v^2 in netlist order (analyzeStability,
+// physics/circuitsim.h:289-295)
+__device__ double cap_energy(const DevTopo& T, const DevBatch& P, int b, const double* v) {
+    const int n = T.n;
+    return block_ordered_sum(T.nC, [&](int i) {
+#pragma clang fp contract(off)
+        double x = 0.0;
+        const int p = T.cap_n1[i], q = T.cap_n2[i];
+        if (p > 0 && p <= n) x += v[p - 1];
+        if (q > 0 && q <= n) x -= v[q - 1];
+        return 0.5 * P.cap[(size_t)b * T.nC + i] * x * x;
+    });
+}
+
+// linear part of A (resistors, Norton conductances, capacitor companions of
+// the instance's current step and method) into glin, by the whole block
+__device__ void lin_matrix(const DevTopo& T, const DevBatch& P, int b) {
+#pragma clang fp contract(off)
+    for (int s = threadIdx.x; s < T.nnz_a; s += blockDim.x) {
+        double a = 0.0;
+        for (int k = T.lin_ptr[s]; k < T.lin_ptr[s + 1]; ++k) {
+            const int t = T.lin_term[k];
+            const int idx = term_idx(t);
+            double g;
+            switch (term_kind(t)) {
+            case karam::SK_RES: g = 1.0 / P.R[(size_t)b * T.nR + idx]; break;
+            case karam::SK_CAP: g = cap_conductance(P, b, T.nC, idx); break;
+            default: g = 1.0e3; break;
+            }
+            a = term_neg(t) ? a + (-g) : a + g;
+        }
+        P.glin[IX(s, b, P.Bp)] = a;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -475,10 +601,14 @@ __global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P) {
 // Start state of one instance (one block per instance) and the per-instance
 // linear part of A and rhs.  mode 0 (DC): seedInitialGuess
 // (solvers/circuitsim_solver.cpp:32-116) or the caller's seed; mode 1
-// (transient): zero voltages; mode 2 (transient): the caller's seed.  In a
-// transient the capacitor companions C/h join the linear part and the
-// capacitor history starts at the start state; the rhs is set per step.
-__global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_step, int mode) {
+// (transient): zero voltages; mode 2 (transient): the caller's seed.  A
+// transient also starts its step control (time 0, step h, method), its
+// capacitor history (branch voltages of the start state, zero current) and
+// its arbiter (the start state, prev_dt = 1e-9, energy 0 from the zero
+// start as IntegratorArbiter::initialize sets it, physics/circuitsim.h:
+// 135-173, or the seed's stored energy); the rhs is set per attempt.
+__global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_step, int mode, double h,
+                                              int method) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
     const int n = T.n, Bp = P.Bp;
@@ -503,25 +633,29 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     }
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) P.vI[IX(i, b, Bp)] = v[i];
-    for (int s = threadIdx.x; s < T.nnz_a; s += blockDim.x) {
-        double a = 0.0;
-        for (int k = T.lin_ptr[s]; k < T.lin_ptr[s + 1]; ++k) {
-            const int t = T.lin_term[k];
-            const int idx = term_idx(t);
-            double g;
-            switch (term_kind(t)) {
-            case karam::SK_RES: g = 1.0 / P.R[(size_t)b * T.nR + idx]; break;
-            case karam::SK_CAP: g = P.cap[(size_t)b * T.nC + idx] / P.h; break;
-            default: g = 1.0e3; break;
-            }
-            a = term_neg(t) ? a + (-g) : a + g;
-        }
-        P.glin[IX(s, b, Bp)] = a;
-    }
     if (mode != 0) {
-        for (int i = threadIdx.x; i < T.nC; i += blockDim.x)
-            P.cap_vprev[(size_t)b * T.nC + i] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
+        for (int i = threadIdx.x; i < T.nC; i += blockDim.x) {
+            const size_t ci = (size_t)b * T.nC + i;
+            P.hv0[ci] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
+            P.hv1[ci] = P.hv0[ci];
+            P.hi0[ci] = 0.0;
+        }
+        if (P.arb_prev)
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                P.arb_prev[(size_t)b * n + i] = v[i];
+                P.arb_prev2[(size_t)b * n + i] = v[i];
+            }
+        const double e0 = use_seed ? cap_energy(T, P, b, v) : 0.0;
+        if (threadIdx.x == 0) {
+            TranState ts;
+            ts.t = 0.0; ts.h = h; ts.prev_energy = e0; ts.prev_dt = 1e-9; ts.glin_h = h;
+            ts.rec = 0; ts.method = method; ts.attempt = 0; ts.instab = 0; ts.accepted = 0; ts.rejected = 0;
+            ts.switched_at = -1; ts.glin_method = method; ts.dec = 0;
+            P.ts[b] = ts;
+        }
+        __syncthreads();
     }
+    lin_matrix(T, P, b);
     if (mode == 0 && P.p32) residual_to_interleaved(T, P, b, v);
     else if (mode == 0)
         for (int i = threadIdx.x; i < n; i += blockDim.x) P.rhslin[IX(i, b, Bp)] = rhs_lin_node(T, P, b, i);
@@ -537,51 +671,246 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     }
 }
 
-// Transient step set-up (one block per instance): source values at t_{n+1}
-// (PULSE sources from their waveform, the others at their DC value) and the
-// linear rhs with the capacitor history.
-__global__ __launch_bounds__(256) void k_step_begin(DevTopo T, DevBatch P, double t_next) {
+// Transient attempt set-up (one block per running instance): source values
+// at t + h (PULSE / SINE sources from their waveform, the others at their
+// DC value), the linear part of A when the step or the method changed, the
+// interleaved copy of the Newton estimate, and the linear rhs with the
+// capacitor history (FP32 path: the KCL vector at the estimate).
+__global__ __launch_bounds__(256) void k_tran_begin(DevTopo T, DevBatch P) {
 #pragma clang fp contract(off)
     const int b = blockIdx.x;
+    if (P.st[b].done != DONE_RUNNING) return;
+    const TranState ts = P.ts[b];
+    const double tn = ts.t + ts.h;
     for (int i = threadIdx.x; i < T.nV; i += blockDim.x)
         P.vsrc_t[(size_t)b * T.nV + i] =
-            T.vsrc_is_pulse[i] ? pulse_value(T.vsrc_pulse + 7 * i, t_next) : P.vsrc[(size_t)b * T.nV + i];
+            wave_value(T.vsrc_wave[i], T.vsrc_wave_par + 7 * i, P.vsrc[(size_t)b * T.nV + i], tn);
+    for (int i = threadIdx.x; i < T.nI; i += blockDim.x)
+        P.isrc_t[(size_t)b * T.nI + i] =
+            wave_value(T.isrc_wave[i], T.isrc_wave_par + 7 * i, P.isrc[(size_t)b * T.nI + i], tn);
+    if (ts.glin_h != ts.h || ts.glin_method != ts.method) {
+        lin_matrix(T, P, b);
+        if (threadIdx.x == 0) {
+            P.ts[b].glin_h = ts.h;
+            P.ts[b].glin_method = ts.method;
+        }
+    }
+    const double* v = P.v + (size_t)b * T.n;
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.vI[IX(i, b, P.Bp)] = v[i];
     __syncthreads();
-    if (P.p32) residual_to_interleaved(T, P, b, P.v + (size_t)b * T.n);
+    if (P.p32) residual_to_interleaved(T, P, b, v);
     else
         for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
 }
 
-// Transient step close (one block per instance): an instance whose step
-// converged shifts its capacitor history, records its probes and is
-// re-armed for the next step; after the last step it is finished.
-__global__ __launch_bounds__(256) void k_step_end(DevTopo T, DevBatch P, int step, int steps, int n_probe,
-                                                  const int* __restrict__ probe, double* __restrict__ wave) {
+__device__ __forceinline__ double lte_norm(double v) {
+#pragma clang fp contract(off)
+    return 1e-6 + fabs(v) * 1e-3;
+}
+
+// calculateLTE (physics/circuitsim.h:177-246) of instance b at its Newton
+// solution v: max over nodes, capacitor branches and MOSFET gate drives of
+// the linear-predictor error times dt / (dt + prev_dt) over
+// 1e-6 + 1e-3 |v|.  The maximum keeps std::max's order (a NaN term is
+// skipped), so it does not depend on how the block splits the terms.
+__device__ double block_lte(const DevTopo& T, const DevBatch& P, int b, const double* v, const TranState& ts,
+                            double* red) {
+#pragma clang fp contract(off)
+    const int n = T.n;
+    const double dt = ts.h, pdt = ts.prev_dt;
+    if (!(dt > 0) || !(pdt > 0)) return 0.0;
+    const double c_factor = dt / (dt + pdt);
+    const double* p1 = P.arb_prev + (size_t)b * n;
+    const double* p2 = P.arb_prev2 + (size_t)b * n;
+    double mx = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v_pred = p1[i] + (p1[i] - p2[i]) * (dt / pdt);
+        const double err = fabs(v[i] - v_pred) * c_factor;
+        mx = ref_max(mx, err / lte_norm(v[i]));
+    }
+    for (int i = threadIdx.x; i < T.nC; i += blockDim.x) {
+        const size_t ci = (size_t)b * T.nC + i;
+        const int p = T.cap_n1[i], q = T.cap_n2[i];
+        const double v1 = p > 0 ? v[p - 1] : 0.0;
+        const double v2 = q > 0 ? v[q - 1] : 0.0;
+        const double v_branch = v1 - v2;
+        const double v_pred = P.hv0[ci] + (P.hv0[ci] - P.hv1[ci]) * (dt / pdt);
+        const double err = fabs(v_branch - v_pred) * c_factor;
+        mx = ref_max(mx, err / lte_norm(v_branch));
+    }
+    for (int i = threadIdx.x; i < T.nM; i += blockDim.x) {
+        const int g = T.m_g[i], so = T.m_s[i];
+        const double vgs = volt(v, n, g) - volt(v, n, so);
+        const double vgs_p = volt(p1, n, g) - volt(p1, n, so);
+        const double vgs_pp = volt(p2, n, g) - volt(p2, n, so);
+        const double vgs_pred = vgs_p + (vgs_p - vgs_pp) * (dt / pdt);
+        const double err = fabs(vgs - vgs_pred) * c_factor;
+        mx = ref_max(mx, err / lte_norm(vgs));
+    }
+    return block_max(mx, red);
+}
+
+// Transient step control after an attempt's Newton solve (one block per
+// instance whose attempt ended).  The restated stepTransient
+// (solvers/circuitsim_solver.cpp:559-780) as its caller drives it
+// (engine_impl.cpp:205-256):
+//   - failed Newton solve: the fixed control ends the instance; the arbiter
+//     halves h (at most 4 attempts a step);
+//   - converged, arbiter control: analyzeStability (physics/circuitsim.h:
+//     276-367): LTE > 1 or energy growth beyond E_prev + 1e-9 + 1.5 P_in h
+//     + 1e-6 and 1.5 E_prev counts an instability: h halves, or at 3 or more
+//     the method switches from trapezoidal to Gear-2 (same h), or, already
+//     switched, the step is taken anyway; a stable step decrements the
+//     counter and becomes the arbiter's new state (its rejectHistory never
+//     alters the estimate, see the oracle);
+//   - accepted: capacitor history shifted with the method's current
+//     (:668-699), t += h, waveform ring entry every `decimation` steps, the
+//     next step starts at h from the arbiter's voltages, or the instance is
+//     finished (`steps` steps, or t >= t_stop).
+__global__ __launch_bounds__(256) void k_tran_control(DevTopo T, DevBatch P, int control, int steps, double h0,
+                                                      double t_stop, int decimation, int n_probe,
+                                                      const int* __restrict__ probe, int wave_cap,
+                                                      double* __restrict__ wave_t, double* __restrict__ wave) {
+#pragma clang fp contract(off)
+    __shared__ double red[8];
     const int b = blockIdx.x;
     const int n = T.n;
     InstState st = P.st[b];
-    if (st.done != 2) return;
-    const double* v = P.v + (size_t)b * n;
-    for (int i = threadIdx.x; i < T.nC; i += blockDim.x)
-        P.cap_vprev[(size_t)b * T.nC + i] = volt(v, n, T.cap_n1[i]) - volt(v, n, T.cap_n2[i]);
-    double* w = wave + ((size_t)b * (steps + 1) + step + 1) * n_probe;
-    for (int p = threadIdx.x; p < n_probe; p += blockDim.x) w[p] = volt(v, n, probe[p]);
+    if (st.done != DONE_CONVERGED && st.done != DONE_FAILED) return;
+    TranState ts = P.ts[b];
+    double* v = P.v + (size_t)b * n;
+    const bool arbiter = control == NR_TRAN_ARBITER;
+    enum { ACCEPT, RETRY, FAIL };
+    int decision;
+    if (st.done == DONE_FAILED) {
+        if (!arbiter) {
+            decision = FAIL;
+        } else {
+            ts.rejected++;
+            ts.h *= 0.5;
+            ts.attempt++;
+            decision = ts.attempt >= 4 ? FAIL : RETRY;
+        }
+    } else if (!arbiter) {
+        decision = ACCEPT;
+    } else {
+        const double lte = block_lte(T, P, b, v, ts, red);
+        const double energy = cap_energy(T, P, b, v);
+        const double power_in = block_ordered_sum(T.nI, [&](int i) {
+#pragma clang fp contract(off)
+            double v_diff = 0.0;
+            const int p = T.isrc_pos[i], q = T.isrc_neg[i];
+            if (p > 0 && p <= n) v_diff += v[p - 1];
+            if (q > 0 && q <= n) v_diff -= v[q - 1];
+            return fabs(P.isrc[(size_t)b * T.nI + i] * v_diff);
+        });
+        const double tol = 1e-9 + power_in * ts.h * 1.5;
+        const bool exploding = (energy > ts.prev_energy + tol + 1e-6) && (energy > ts.prev_energy * 1.5);
+        int sug;
+        if (lte > 1.0 || exploding) {
+            ts.instab++;
+            sug = ts.instab >= 3 ? 2 : 1;
+        } else {
+            ts.instab = max(0, ts.instab - 1);
+            double* p1 = P.arb_prev + (size_t)b * n;
+            double* p2 = P.arb_prev2 + (size_t)b * n;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                p2[i] = p1[i];
+                p1[i] = v[i];
+            }
+            ts.prev_energy = energy;
+            ts.prev_dt = ts.h;
+            sug = 0;
+        }
+        if (sug == 1) {
+            ts.rejected++;
+            ts.h *= 0.5;
+            ts.attempt++;
+            decision = ts.attempt >= 4 ? FAIL : RETRY;
+        } else if (sug == 2 && ts.method == NR_INT_TRAP) {
+            ts.rejected++;
+            ts.method = NR_INT_GEAR2;
+            ts.switched_at = ts.accepted;
+            decision = RETRY;
+        } else {
+            decision = ACCEPT;
+        }
+    }
+    if (decision == FAIL) ts.h = P.ts[b].h;   // the failed attempt's step
+    bool finished = false;
+    if (decision == ACCEPT) {
+        for (int i = threadIdx.x; i < T.nC; i += blockDim.x) {
+            const size_t ci = (size_t)b * T.nC + i;
+            double vc = 0.0;
+            const int p = T.cap_n1[i], q = T.cap_n2[i];
+            if (p > 0 && p <= n) vc += v[p - 1];
+            if (q > 0 && q <= n) vc -= v[q - 1];
+            const double C = P.cap[ci], h = ts.h, v0 = P.hv0[ci], v1 = P.hv1[ci];
+            double ic;
+            if (ts.method == NR_INT_TRAP) ic = (2.0 * C / h) * (vc - v0) - P.hi0[ci];
+            else if (ts.method == NR_INT_GEAR1) ic = (C / h) * (vc - v0);
+            else if (v1 == 0.0 && ts.t > h) ic = (C / h) * (vc - v0);
+            else ic = (C / h) * (1.5 * vc - 2.0 * v0 + 0.5 * v1);
+            P.hv1[ci] = v0;
+            P.hv0[ci] = vc;
+            P.hi0[ci] = ic;
+        }
+        ts.t = ts.t + ts.h;
+        ts.accepted++;
+        if (++ts.dec >= decimation) {
+            ts.dec = 0;
+            const long long slot = ts.rec % wave_cap;
+            if (threadIdx.x == 0) wave_t[(size_t)b * wave_cap + slot] = ts.t;
+            double* w = wave + ((size_t)b * wave_cap + slot) * n_probe;
+            for (int p = threadIdx.x; p < n_probe; p += blockDim.x) w[p] = volt(v, n, probe[p]);
+            ts.rec++;
+        }
+        finished = arbiter ? !(ts.t < t_stop) : ts.accepted >= steps;
+        if (!finished) {
+            ts.h = h0;
+            ts.attempt = 0;
+        }
+    }
+    // the next attempt starts from the arbiter's voltages (the fixed control
+    // continues from the accepted step)
+    if (arbiter && (decision == RETRY || (decision == ACCEPT && !finished))) {
+        __syncthreads();
+        const double* p1 = P.arb_prev + (size_t)b * n;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = p1[i];
+    }
     if (threadIdx.x == 0) {
-        st.step = step + 1;
-        st.it = 0;
-        st.done = (step + 1 == steps) ? 1 : 0;
-        if (st.done) st.status = NR_STATUS_CONVERGED;
+        st.step = ts.accepted;
+        if (decision == FAIL) {
+            st.done = DONE_FINISHED;
+            if (st.status == NR_STATUS_CONVERGED) st.status = NR_STATUS_NOT_CONVERGED;
+        } else if (decision == ACCEPT && finished) {
+            st.done = DONE_FINISHED;
+            st.status = NR_STATUS_CONVERGED;
+        } else {
+            if (decision == RETRY && st.done == DONE_CONVERGED) st.status = NR_STATUS_NOT_CONVERGED;
+            st.done = DONE_RUNNING;
+            st.it = 0;
+            st.singular = 0;
+            st.prev_kcl = 0.0;
+            if (decision == ACCEPT) st.status = NR_STATUS_NOT_CONVERGED;
+        }
         P.st[b] = st;
+        P.ts[b] = ts;
     }
 }
 
-// probes of the start state
-__global__ __launch_bounds__(64) void k_wave_start(DevTopo T, DevBatch P, int steps, int n_probe,
-                                                   const int* __restrict__ probe, double* __restrict__ wave) {
+// waveform ring entry 0: the start state at t = 0
+__global__ __launch_bounds__(64) void k_wave_start(DevTopo T, DevBatch P, int n_probe, const int* __restrict__ probe,
+                                                   int wave_cap, double* __restrict__ wave_t,
+                                                   double* __restrict__ wave) {
     const int b = blockIdx.x;
     const double* v = P.v + (size_t)b * T.n;
-    double* w = wave + (size_t)b * (steps + 1) * n_probe;
+    double* w = wave + (size_t)b * wave_cap * n_probe;
     for (int p = threadIdx.x; p < n_probe; p += blockDim.x) w[p] = volt(v, T.n, probe[p]);
+    if (threadIdx.x == 0) {
+        wave_t[(size_t)b * wave_cap] = 0.0;
+        P.ts[b].rec = 1;
+    }
 }
 
 // thread (item, lane): 4 items x 64 instances per block; blockIdx.y = group
@@ -1356,9 +1685,9 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
 // kernels: NR bookkeeping
 // ---------------------------------------------------------------------------
 // Per-instance NR control after a linear solve (one block per instance).
-// tran = 1: a transient step (no limiting, no damping, KCL_TOL = 1e-12, a
-// converged instance waits for the next step; solvers/circuitsim_solver.cpp:
-// 600-635 restated).  On the FP32 path the solve returned the correction,
+// tran = 1: a transient attempt (no limiting, no damping, KCL_TOL = 1e-12;
+// solvers/circuitsim_solver.cpp:600-635 restated); a converged or failed
+// attempt leaves the active list and k_tran_control decides.  On the FP32 path the solve returned the correction,
 // so the new iterate is v + x (FP64), and every KCL audit also stores its
 // per-node vector (the next Newton rhs).
 __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
@@ -1378,7 +1707,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
             st.prev_kcl = 1e9;
             if (tran) st.kcl = 1e9;
             st.status = NR_STATUS_SINGULAR;
-            st.done = 1;
+            st.done = tran ? DONE_FAILED : DONE_FINISHED;
             P.st[b] = st;
         }
         return;
@@ -1443,12 +1772,12 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         if (conv) {
             st.total += st.it + 1;
             st.ref += st.it + 1;
-            st.done = 2;                      // step converged
+            st.done = DONE_CONVERGED;         // the step control decides
         } else if (++st.it >= max_iter) {
             st.total += max_iter;
             st.ref += max_iter;
             st.status = NR_STATUS_NOT_CONVERGED;
-            st.done = 1;
+            st.done = DONE_FAILED;
         }
         P.st[b] = st;
     } else if (threadIdx.x == 0) {

@@ -220,6 +220,14 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool tra
         if (bad_node(nl->vsrc_pos[i]) || bad_node(nl->vsrc_neg[i])) { err = "vsource node out of range"; return NR_ERR_INVALID; }
     for (int32_t i = 0; i < nl->n_isrc; ++i)
         if (bad_node(nl->isrc_pos[i]) || bad_node(nl->isrc_neg[i])) { err = "isource node out of range"; return NR_ERR_INVALID; }
+    auto bad_wave = [](int32_t count, const int32_t* kind) {
+        for (int32_t i = 0; kind && i < count; ++i)
+            if (kind[i] < NR_WAVE_DC || kind[i] > NR_WAVE_SINE) return true;
+        return false;
+    };
+    if (bad_wave(nl->n_vsrc, nl->vsrc_wave) || bad_wave(nl->n_isrc, nl->isrc_wave)) {
+        err = "unknown source waveform kind"; return NR_ERR_INVALID;
+    }
     for (int32_t i = 0; i < nl->n_cap; ++i)
         if (bad_node(nl->cap_n1[i]) || bad_node(nl->cap_n2[i])) { err = "capacitor node out of range"; return NR_ERR_INVALID; }
     for (int32_t i = 0; i < nl->n_diode; ++i)
@@ -245,12 +253,17 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool tra
     t.cap_n1.assign(nl->cap_n1, nl->cap_n1 + nl->n_cap);
     t.cap_n2.assign(nl->cap_n2, nl->cap_n2 + nl->n_cap);
     t.cap_farads.assign(nl->cap_farads, nl->cap_farads + nl->n_cap);
-    t.vsrc_is_pulse.assign(nl->n_vsrc, 0);
-    t.vsrc_pulse.assign((size_t)nl->n_vsrc * 7, 0.0);
-    for (int32_t i = 0; i < nl->n_vsrc && nl->vsrc_is_pulse && nl->vsrc_pulse; ++i) {
-        t.vsrc_is_pulse[i] = nl->vsrc_is_pulse[i] ? 1 : 0;
-        for (int q = 0; q < 7; ++q) t.vsrc_pulse[(size_t)i * 7 + q] = nl->vsrc_pulse[(size_t)i * 7 + q];
-    }
+    auto waves = [](int32_t count, const int32_t* kind, const double* par, std::vector<int32_t>& k,
+                    std::vector<double>& p) {
+        k.assign(count, NR_WAVE_DC);
+        p.assign((size_t)count * 7, 0.0);
+        for (int32_t i = 0; i < count && kind && par; ++i) {
+            k[i] = kind[i];
+            for (int q = 0; q < 7; ++q) p[(size_t)i * 7 + q] = par[(size_t)i * 7 + q];
+        }
+    };
+    waves(nl->n_vsrc, nl->vsrc_wave, nl->vsrc_wave_par, t.vsrc_wave, t.vsrc_wave_par);
+    waves(nl->n_isrc, nl->isrc_wave, nl->isrc_wave_par, t.isrc_wave, t.isrc_wave_par);
     t.transient = transient;
     tensorize(nl, t.soa);
     const SoA& s = t.soa;

@@ -82,8 +82,8 @@ struct Topology {
     int32_t n_res = 0, n_vsrc = 0, n_isrc = 0, n_cap = 0, n_dio_nl = 0, n_mos_nl = 0;
     std::vector<int32_t> res_n1, res_n2, vsrc_pos, vsrc_neg, isrc_pos, isrc_neg, dio_anode_nl;
     std::vector<double> res_ohms, vsrc_volts, isrc_amps;
-    std::vector<int32_t> cap_n1, cap_n2, vsrc_is_pulse;
-    std::vector<double> cap_farads, vsrc_pulse;   // pulse: 7 values per source
+    std::vector<int32_t> cap_n1, cap_n2, vsrc_wave, isrc_wave;   // NR_WAVE_* per source
+    std::vector<double> cap_farads, vsrc_wave_par, isrc_wave_par;  // 7 values per source
     // transient topology: capacitor companion stamps are part of the pattern
     bool transient = false;
 

@@ -23,6 +23,7 @@ from . import _abi
 
 VT_DEFAULT = 0.02585          # addDiode default thermal voltage (circuit.h:815)
 AMBIENT_K = 300.15            # SimulationEnvironment default
+WAVE_DC, WAVE_PULSE, WAVE_SINE = _abi.NR_WAVE_DC, _abi.NR_WAVE_PULSE, _abi.NR_WAVE_SINE
 
 
 def _i32(x):
@@ -55,11 +56,13 @@ class Netlist:
     vsrc_pos: np.ndarray = field(default_factory=lambda: _i32([]))
     vsrc_neg: np.ndarray = field(default_factory=lambda: _i32([]))
     vsrc_volts: np.ndarray = field(default_factory=lambda: _f64([]))
-    vsrc_is_pulse: np.ndarray = field(default_factory=lambda: _i32([]))
-    vsrc_pulse: np.ndarray = field(default_factory=lambda: _f64(np.zeros((0, 7))))
+    vsrc_wave: np.ndarray = field(default_factory=lambda: _i32([]))
+    vsrc_wave_par: np.ndarray = field(default_factory=lambda: _f64(np.zeros((0, 7))))
     isrc_pos: np.ndarray = field(default_factory=lambda: _i32([]))
     isrc_neg: np.ndarray = field(default_factory=lambda: _i32([]))
     isrc_amps: np.ndarray = field(default_factory=lambda: _f64([]))
+    isrc_wave: np.ndarray = field(default_factory=lambda: _i32([]))
+    isrc_wave_par: np.ndarray = field(default_factory=lambda: _f64(np.zeros((0, 7))))
     dio_anode: np.ndarray = field(default_factory=lambda: _i32([]))
     dio_cathode: np.ndarray = field(default_factory=lambda: _i32([]))
     dio_is: np.ndarray = field(default_factory=lambda: _f64([]))
@@ -152,11 +155,14 @@ class Netlist:
         nl.cap_n1, nl.cap_n2, nl.cap_farads = p32(self.cap_n1), p32(self.cap_n2), p64(self.cap_farads)
         nl.n_vsrc = len(self.vsrc_volts)
         nl.vsrc_pos, nl.vsrc_neg, nl.vsrc_volts = p32(self.vsrc_pos), p32(self.vsrc_neg), p64(self.vsrc_volts)
-        if len(self.vsrc_is_pulse) == nl.n_vsrc and nl.n_vsrc:
-            nl.vsrc_is_pulse = p32(self.vsrc_is_pulse)
-            nl.vsrc_pulse = p64(np.asarray(self.vsrc_pulse).reshape(-1))
+        if len(self.vsrc_wave) == nl.n_vsrc and nl.n_vsrc:
+            nl.vsrc_wave = p32(self.vsrc_wave)
+            nl.vsrc_wave_par = p64(np.asarray(self.vsrc_wave_par).reshape(-1))
         nl.n_isrc = len(self.isrc_amps)
         nl.isrc_pos, nl.isrc_neg, nl.isrc_amps = p32(self.isrc_pos), p32(self.isrc_neg), p64(self.isrc_amps)
+        if len(self.isrc_wave) == nl.n_isrc and nl.n_isrc:
+            nl.isrc_wave = p32(self.isrc_wave)
+            nl.isrc_wave_par = p64(np.asarray(self.isrc_wave_par).reshape(-1))
         nl.n_diode = len(self.dio_is)
         nl.dio_anode, nl.dio_cathode = p32(self.dio_anode), p32(self.dio_cathode)
         nl.dio_is, nl.dio_n, nl.dio_vt = p64(self.dio_is), p64(self.dio_n), p64(self.dio_vt)
@@ -194,7 +200,7 @@ def inverter_chain(stages: int = 16, vdd: float = 1.8, cload: float = 10e-15,
         num_nodes=n,
         cap_n1=_i32(range(3, n + 1)), cap_n2=_i32([0] * stages), cap_farads=_f64([cload] * stages),
         vsrc_pos=_i32([1, 2]), vsrc_neg=_i32([0, 0]), vsrc_volts=_f64([vdd, pulse[0]]),
-        vsrc_is_pulse=_i32([0, 1]), vsrc_pulse=_f64([[0] * 7, list(pulse)]),
+        vsrc_wave=_i32([0, WAVE_PULSE]), vsrc_wave_par=_f64([[0] * 7, list(pulse)]),
         mos_d=_i32(md), mos_g=_i32(mg), mos_s=_i32(ms), mos_b=_i32(mb),
         mos_w=_f64(mw), mos_l=_f64(ml), mos_model=_i32(mm),
         models=[NMOS_CARD, PMOS_CARD], name=f"inv{stages}",
@@ -357,17 +363,17 @@ def random_logic(n_target: int = 20000, seed: int = 1, n_inputs: int = 32, windo
     pi_v = rng.integers(0, 2, n_inputs) * vdd
     n_pulse = min(n_pulse, n_inputs)
     pulse = np.zeros((1 + n_inputs, 7))
-    is_pulse = np.zeros(1 + n_inputs, dtype=np.int32)
+    wave = np.zeros(1 + n_inputs, dtype=np.int32)
     for k in range(n_pulse):
         per = rng.uniform(20e-9, 60e-9)
         pulse[1 + k] = [0.0, vdd, rng.uniform(1e-9, 10e-9), 0.5e-9, 0.5e-9, rng.uniform(0.3, 0.6) * per, per]
-        is_pulse[1 + k] = 1
+        wave[1 + k] = WAVE_PULSE
         pi_v[k] = 0.0
     return Netlist(
         num_nodes=node,
         cap_n1=_i32(cap_n), cap_n2=_i32(np.zeros(len(cap_n))), cap_farads=_f64(np.full(len(cap_n), cload)),
         vsrc_pos=_i32([1] + pis), vsrc_neg=_i32(np.zeros(1 + n_inputs)), vsrc_volts=_f64([vdd] + list(pi_v)),
-        vsrc_is_pulse=_i32(is_pulse), vsrc_pulse=_f64(pulse),
+        vsrc_wave=_i32(wave), vsrc_wave_par=_f64(pulse),
         mos_d=_i32(md), mos_g=_i32(mg), mos_s=_i32(ms), mos_b=_i32(mb),
         mos_w=_f64(mw), mos_l=_f64(ml), mos_model=_i32(mm),
         models=[NMOS_CARD, PMOS_CARD], name=f"logic{node}",

@@ -39,7 +39,8 @@ EXPORTED = [
     "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
     "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
     "nr_eval_diodes", "nr_eval_mosfets",
-    "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform", "nr_batch_autotune",
+    "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform", "nr_batch_get_waveform_times",
+    "nr_batch_get_wave_recorded", "nr_batch_get_tran_stats", "nr_batch_autotune",
     "nr_seeder_create", "nr_seeder_destroy", "nr_seeder_record", "nr_seeder_history", "nr_seeder_reset",
     "nr_train_opts_default", "nr_seeder_train", "nr_seeder_predict", "nr_batch_set_seed_from",
     "nr_seeder_param_count", "nr_seeder_get_params", "nr_seeder_set_params", "nr_seeder_loss_grad",
@@ -68,6 +69,9 @@ def lib():
         L.nr_batch_transient.argtypes = [_vp, C.POINTER(_abi.NrTransientOpts), C.POINTER(_abi.NrResult)]
         L.nr_batch_autotune.argtypes = [_vp, C.c_int32, C.c_int32, C.POINTER(_abi.NrAutotuneResult)]
         L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_waveform_times.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_wave_recorded.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int64]
+        L.nr_batch_get_tran_stats.argtypes = [_vp, C.POINTER(_abi.NrTranStats), C.c_int64]
         _f32p = C.POINTER(C.c_float)
         L.nr_seeder_create.argtypes = [_vp, C.POINTER(_abi.NrSeederCfg), C.POINTER(_vp)]
         L.nr_seeder_destroy.argtypes = [_vp]
@@ -294,9 +298,12 @@ class Batch:
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
 
     def transient(self, h=1e-9, steps=200, max_iter=50, start=_abi.NR_TRAN_START_ZERO, probes=None,
-                  record_timing=0, precision=_abi.NR_PREC_FP64, use_graph=0):
-        """Fixed-step backward-Euler transient of every instance (needs a
-        transient topology).  probes: 1-based node ids recorded each step."""
+                  record_timing=0, precision=_abi.NR_PREC_FP64, use_graph=0, method=_abi.NR_INT_GEAR1,
+                  control=_abi.NR_TRAN_FIXED, t_stop=0.0, wave_capacity=0, wave_decimation=1):
+        """Transient of every instance (needs a transient topology): `steps`
+        fixed steps of h, or with control=NR_TRAN_ARBITER the reference's
+        step control up to t_stop.  probes: 1-based node ids recorded in the
+        waveform ring."""
         o = _abi.NrTransientOpts()
         lib().nr_transient_opts_default(C.byref(o))
         self._probes = np.ascontiguousarray(probes if probes is not None else [], dtype=np.int32)
@@ -306,9 +313,15 @@ class Batch:
         o.record_timing = record_timing
         o.precision = int(precision)
         o.use_graph = int(use_graph)
+        o.method, o.control, o.t_stop = int(method), int(control), float(t_stop)
+        o.wave_capacity, o.wave_decimation = int(wave_capacity), int(wave_decimation)
         r = _abi.NrResult()
         _check(lib().nr_batch_transient(self.h, C.byref(o), C.byref(r)), self.ctx)
-        self._wave_shape = (self.B, int(steps) + 1, len(self._probes))
+        ring = int(wave_capacity)
+        if ring == 0:
+            ring = (int(np.ceil(float(t_stop) / float(h))) + 1 if int(control) == _abi.NR_TRAN_ARBITER
+                    else int(steps) + 1)
+        self._wave_shape = (self.B, ring, len(self._probes))
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
 
     def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
@@ -323,10 +336,30 @@ class Batch:
                 "ms_before": r.ms_before, "ms_after": r.ms_after}
 
     def waveform(self):
-        """Probe voltages of the last transient, [B, steps + 1, n_probe]."""
+        """Probe voltages of the last transient's waveform ring,
+        [B, capacity, n_probe] (entry k in slot k mod capacity)."""
         out = np.zeros(self._wave_shape)
         _check(lib().nr_batch_get_waveform(self.h, _p64(out), out.size), self.ctx)
         return out
+
+    def waveform_times(self):
+        """Times of the waveform ring's entries, [B, capacity]."""
+        out = np.zeros(self._wave_shape[:2])
+        _check(lib().nr_batch_get_waveform_times(self.h, _p64(out), out.size), self.ctx)
+        return out
+
+    def wave_recorded(self):
+        """Entries written to each instance's waveform ring."""
+        out = np.zeros(self.B, dtype=np.int64)
+        _check(lib().nr_batch_get_wave_recorded(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)), self.B),
+               self.ctx)
+        return out
+
+    def tran_stats(self):
+        """Per-instance step control of the last transient."""
+        arr = (_abi.NrTranStats * self.B)()
+        _check(lib().nr_batch_get_tran_stats(self.h, arr, self.B), self.ctx)
+        return {k: np.array([getattr(x, k) for x in arr]) for k, _ in _abi.NrTranStats._fields_ if k != "pad"}
 
     def voltages(self):
         n = self.topo.info["n"]

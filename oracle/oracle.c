@@ -315,13 +315,18 @@ static inline void emit(emitter* e, int32_t r, int32_t c, double val) {
     else { iv_push(&e->rows, r); iv_push(&e->cols, c); }
 }
 
-/* Transient step context (restated stepTransient, solvers/circuitsim_solver.cpp:
- * 559-780): step h, source values at t_{n+1}, capacitor voltages of the last
- * accepted step.  NULL for the DC solve. */
+/* Transient attempt context (restated stepTransient, solvers/circuitsim_solver.cpp:
+ * 559-780): step h, companion method (ORC_INT_* = IntegrationType), source
+ * values at t + h, capacitor history (branch voltages of the last two
+ * accepted steps, current of the last).  NULL for the DC solve. */
 typedef struct {
     double h;
+    int32_t method;
     const double* vsrc;     /* [n_vsrc] */
-    const double* vprev;    /* [n_cap]  */
+    const double* isrc;     /* [n_isrc] */
+    const double* hv0;      /* [n_cap]  */
+    const double* hv1;      /* [n_cap]  */
+    const double* hi0;      /* [n_cap]  */
     int coords_only;        /* pattern build: emit the capacitor coordinates */
 } tran_ctx;
 
@@ -354,30 +359,46 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
             if (q != 0) rhs[q - 1] -= ieq;
         }
     }
-    /* (3) current sources :2469-2477 */
+    /* (3) current sources :2469-2477 (waveform value in a transient) */
     if (rhs) {
         for (int32_t i = 0; i < nl->n_isrc; ++i) {
             int32_t p = nl->isrc_pos[i], q = nl->isrc_neg[i];
-            double I = nl->isrc_amps[i];
+            double I = (tc && tc->isrc) ? tc->isrc[i] : nl->isrc_amps[i];
             if (p != 0) rhs[p - 1] -= I;
             if (q != 0) rhs[q - 1] += I;
         }
     }
-    /* (4) capacitors, backward-Euler companion G = C/h with history current
-     * G v_prev (restating :2479-2505 with the Gear-1 companion of
-     * math/integration_method.h:83-115) */
+    /* (4) capacitors: the companion of the attempt's method (restating
+     * :2479-2505 with math/integration_method.h:47-157).  Trapezoidal: G =
+     * 2C/h, I_eq = -G v0 - i0, rhs[a] -= I_eq, rhs[b] += I_eq; Gear-1: G =
+     * C/h, I_eq = -G v0; Gear-2: G = 1.5C/h, I_src = C (2/h v0 - 0.5/h v1),
+     * rhs[a] += I_src, rhs[b] -= I_src.  The history always holds its three
+     * slots (netlist/circuit.h:1155-1156), so Gear-2's sparse-history
+     * fallback (:131-136) is never taken. */
     for (int32_t i = 0; tc && i < nl->n_cap; ++i) {
         int32_t a = nl->cap_n1[i], b = nl->cap_n2[i];
-        double G = 0.0, ih = 0.0;
-        if (!tc->coords_only) { G = nl->cap_farads[i] / tc->h; ih = G * tc->vprev[i]; }
+        const double C = nl->cap_farads[i];
+        double G = 0.0, ra = 0.0;   /* ra: added to rhs[a], subtracted from rhs[b] */
+        if (!tc->coords_only) {
+            if (tc->method == ORC_INT_TRAP) {
+                G = (2.0 * C) / tc->h;
+                ra = -(-G * tc->hv0[i] - tc->hi0[i]);
+            } else if (tc->method == ORC_INT_GEAR2) {
+                G = (1.5 * C) / tc->h;
+                ra = C * ((2.0 / tc->h) * tc->hv0[i] - (0.5 / tc->h) * tc->hv1[i]);
+            } else {
+                G = C / tc->h;
+                ra = -(-G * tc->hv0[i]);
+            }
+        }
         if (a != 0 && b != 0) {
             emit(e, a - 1, a - 1, G); emit(e, b - 1, b - 1, G);
             emit(e, a - 1, b - 1, -G); emit(e, b - 1, a - 1, -G);
         } else if (a != 0) emit(e, a - 1, a - 1, G);
         else if (b != 0) emit(e, b - 1, b - 1, G);
         if (rhs) {
-            if (a != 0) rhs[a - 1] += ih;
-            if (b != 0) rhs[b - 1] -= ih;
+            if (a != 0) rhs[a - 1] += ra;
+            if (b != 0) rhs[b - 1] -= ra;
         }
     }
     e->n_lin = e->vals ? e->nv : e->rows.n;
@@ -757,7 +778,7 @@ int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** ou
     t->soa = orc_tensorize(nl);
     emitter e;
     memset(&e, 0, sizeof(e));
-    tran_ctx coords = {0.0, NULL, NULL, 1};
+    tran_ctx coords = {0.0, ORC_INT_GEAR1, NULL, NULL, NULL, NULL, NULL, 1};
     stamp_all(nl, t->soa, NULL, 0.0, &e, transient ? &coords : NULL);
     t->n_stamps = e.rows.n;
     t->st_row = e.rows.a; t->st_col = e.cols.a;
@@ -1305,7 +1326,8 @@ static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
         ADDRES(nl->vsrc_pos[i], isrc); ADDRES(nl->vsrc_neg[i], -isrc);
     }
     for (int32_t i = 0; i < nl->n_isrc; ++i) {
-        ADDRES(nl->isrc_pos[i], -nl->isrc_amps[i]); ADDRES(nl->isrc_neg[i], nl->isrc_amps[i]);
+        const double I = (tc && tc->isrc) ? tc->isrc[i] : nl->isrc_amps[i];
+        ADDRES(nl->isrc_pos[i], -I); ADDRES(nl->isrc_neg[i], I);
     }
     for (int32_t i = 0; i < nl->n_diode; ++i) {
         double vd = volt(v, n, nl->dio_anode[i]) - volt(v, n, nl->dio_cathode[i]);
@@ -1324,10 +1346,20 @@ static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
         }
         free(pos);
     }
-    /* capacitor displacement current, Gear-1 branch (physics/circuitsim.h:1460-1490) */
+    /* capacitor displacement current of the attempt's method
+     * (physics/circuitsim.h:1460-1490) */
     for (int32_t i = 0; tc && i < nl->n_cap; ++i) {
-        double vc = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
-        double ic = (nl->cap_farads[i] / tc->h) * (vc - tc->vprev[i]);
+        const double vc = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
+        const double C = nl->cap_farads[i];
+        double ic;
+        if (tc->method == ORC_INT_TRAP) {
+            const double geq = (2.0 * C) / tc->h;
+            ic = geq * (vc - tc->hv0[i]) - tc->hi0[i];
+        } else if (tc->method == ORC_INT_GEAR2) {
+            ic = (C / tc->h) * (1.5 * vc - 2.0 * tc->hv0[i] + 0.5 * tc->hv1[i]);
+        } else {
+            ic = (C / tc->h) * (vc - tc->hv0[i]);
+        }
         ADDRES(nl->cap_n1[i], -ic); ADDRES(nl->cap_n2[i], ic);
     }
 #undef ADDRES
@@ -1499,81 +1531,324 @@ double orc_pulse_value(const double* p, double time) {
     return v1;
 }
 
-void orc_tran_opts_default(orc_tran_opts* o) {
-    o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_MULTIFRONTAL;
-    o->precision = ORC_PREC_FP64;
+/* SINE value at `time` (physics/circuitsim.h:2171-2180);
+ * p = vo va freq td theta phase_deg */
+double orc_sine_value(const double* p, double time) {
+    const double vo = p[0], va = p[1], freq = p[2], td = p[3], theta = p[4], ph = p[5];
+    double t = time - td;
+    if (t > 0) {
+        double omega = 2.0 * 3.141592653589793 * freq;
+        double damping = exp(-t * theta);
+        double phase = ph * (3.141592653589793 / 180.0);
+        return vo + va * sin(omega * t + phase) * damping;
+    }
+    return vo;
 }
 
+double orc_wave_value(int32_t kind, const double* par, double dc, double time) {
+    if (kind == NR_WAVE_PULSE) return orc_pulse_value(par, time);
+    if (kind == NR_WAVE_SINE) return orc_sine_value(par, time);
+    return dc;
+}
+
+void orc_tran_opts_default(orc_tran_opts* o) {
+    o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_MULTIFRONTAL;
+    o->precision = ORC_PREC_FP64; o->method = ORC_INT_GEAR1; o->control = ORC_TRAN_FIXED; o->t_stop = 0.0;
+    o->wave_cap = 0; o->decimation = 1;
+}
+
+int32_t orc_wave_capacity(const orc_tran_opts* o) {
+    if (o->wave_cap > 0) return o->wave_cap;
+    if (o->control == ORC_TRAN_ARBITER) return (int32_t)ceil(o->t_stop / o->h) + 1;
+    return o->steps + 1;
+}
+
+/* IntegratorArbiter (physics/circuitsim.h:57-368), one per instance. */
+typedef struct {
+    double *prev, *prev2, *dvdt;   /* [n]: last two stable solutions, their dV/dt */
+    double prev_energy, prev_dt;
+    int32_t instab, taa_init;
+} orc_arbiter;
+
+/* rejectHistory (:90-121): nodes whose quadratic prediction error
+ * |d2V/dt2| dt^2 exceeds 10 mV get the linear extrapolation; returns how
+ * many.  dV/dt_current and the stored dV/dt are the same quotient of the
+ * same operands (analyzeStability stores (v - prev) / dt together with
+ * prev2 = prev, prev = v, prev_dt = dt), so the acceleration is exactly 0
+ * and nothing is ever rejected; the count checks that. */
+static int32_t arb_reject_history(const orc_arbiter* a, int32_t n, double* pred, double dt) {
+    if (!a->taa_init || dt <= 0 || a->prev_dt <= 0) return 0;
+    int32_t cnt = 0;
+    const double dt_sq = dt * dt;
+    for (int32_t i = 0; i < n; ++i) {
+        double dv_dt_current = (a->prev[i] - a->prev2[i]) / a->prev_dt;
+        double ddv_dt2 = (dv_dt_current - a->dvdt[i]) / (0.5 * (dt + a->prev_dt));
+        double accel_error = fabs(ddv_dt2) * dt_sq;
+        if (accel_error > 0.01) { pred[i] = a->prev[i] + dv_dt_current * dt; ++cnt; }
+    }
+    return cnt;
+}
+
+static inline double lte_norm(double v) { return 1e-6 + fabs(v) * 1e-3; }
+
+/* calculateLTE (:177-246): max over nodes, capacitor branches and MOSFET
+ * gate drives of the linear-predictor error scaled by dt / (dt + prev_dt)
+ * and normalised by 1e-6 + 1e-3 |v|. */
+static double arb_lte(const nr_netlist* nl, const orc_arbiter* a, int32_t n, const double* cur,
+                      const double* hv0, const double* hv1, double dt) {
+    if (dt <= 0 || a->prev_dt <= 0) return 0.0;
+    const double c_factor = dt / (dt + a->prev_dt);
+    double max_err = 0.0;
+    for (int32_t i = 0; i < n; ++i) {
+        double v_pred = a->prev[i] + (a->prev[i] - a->prev2[i]) * (dt / a->prev_dt);
+        double err = fabs(cur[i] - v_pred) * c_factor;
+        max_err = dmax_ref(max_err, err / lte_norm(cur[i]));
+    }
+    for (int32_t i = 0; i < nl->n_cap; ++i) {
+        const int32_t p = nl->cap_n1[i], q = nl->cap_n2[i];
+        double v1 = (p > 0) ? cur[p - 1] : 0.0;
+        double v2 = (q > 0) ? cur[q - 1] : 0.0;
+        double v_branch = v1 - v2;
+        double v_pred = hv0[i] + (hv0[i] - hv1[i]) * (dt / a->prev_dt);
+        double err = fabs(v_branch - v_pred) * c_factor;
+        max_err = dmax_ref(max_err, err / lte_norm(v_branch));
+    }
+    for (int32_t i = 0; i < nl->n_mos; ++i) {
+        const int32_t g = nl->mos_g[i], so = nl->mos_s[i];
+        double vgs = volt(cur, n, g) - volt(cur, n, so);
+        double vgs_p = volt(a->prev, n, g) - volt(a->prev, n, so);
+        double vgs_pp = volt(a->prev2, n, g) - volt(a->prev2, n, so);
+        double vgs_pred = vgs_p + (vgs_p - vgs_pp) * (dt / a->prev_dt);
+        double err = fabs(vgs - vgs_pred) * c_factor;
+        max_err = dmax_ref(max_err, err / lte_norm(vgs));
+    }
+    return max_err;
+}
+
+/* stored capacitor energy sum 0.5 C v^2, netlist order (:289-295) */
+static double cap_energy(const nr_netlist* nl, int32_t n, const double* v) {
+    double e = 0.0;
+    for (int32_t i = 0; i < nl->n_cap; ++i) {
+        double x = 0.0;
+        if (nl->cap_n1[i] > 0 && nl->cap_n1[i] <= n) x += v[nl->cap_n1[i] - 1];
+        if (nl->cap_n2[i] > 0 && nl->cap_n2[i] <= n) x -= v[nl->cap_n2[i] - 1];
+        e += 0.5 * nl->cap_farads[i] * x * x;
+    }
+    return e;
+}
+
+/* analyzeStability (:276-367): 0 stable (the arbiter takes the solution),
+ * 1 reduce the step, 2 switch the method.  The source power uses the DC
+ * current value, as the reference does. */
+static int arb_analyze(const nr_netlist* nl, orc_arbiter* a, int32_t n, const double* cur, const double* hv0,
+                       const double* hv1, double dt) {
+    double lte = arb_lte(nl, a, n, cur, hv0, hv1, dt);
+    int high_lte = (lte > 1.0);
+    double current_energy = cap_energy(nl, n, cur);
+    double power_in = 0.0;
+    for (int32_t i = 0; i < nl->n_isrc; ++i) {
+        double v_diff = 0.0;
+        if (nl->isrc_pos[i] > 0 && nl->isrc_pos[i] <= n) v_diff += cur[nl->isrc_pos[i] - 1];
+        if (nl->isrc_neg[i] > 0 && nl->isrc_neg[i] <= n) v_diff -= cur[nl->isrc_neg[i] - 1];
+        power_in += fabs(nl->isrc_amps[i] * v_diff);
+    }
+    double energy_tolerance = 1e-9 + power_in * dt * 1.5;
+    int exploding = (current_energy > a->prev_energy + energy_tolerance + 1e-6) &&
+                    (current_energy > a->prev_energy * 1.5);
+    if (high_lte || exploding) {
+        a->instab++;
+        return a->instab >= 3 ? 2 : 1;
+    }
+    a->instab = a->instab - 1 > 0 ? a->instab - 1 : 0;
+    if (a->prev_dt > 0) {
+        for (int32_t i = 0; i < n; ++i) a->dvdt[i] = (cur[i] - a->prev[i]) / dt;
+        a->taa_init = 1;
+    }
+    memcpy(a->prev2, a->prev, sizeof(double) * (size_t)n);
+    memcpy(a->prev, cur, sizeof(double) * (size_t)n);
+    a->prev_energy = current_energy;
+    a->prev_dt = dt;
+    return 0;
+}
+
+/* analyzeStability on a given state (for tests): prev / prev2 [n],
+ * scalars in/out; returns the suggestion, *lte_out the LTE */
+int orc_arbiter_analyze(const nr_netlist* nl, const double* cur, const double* hv0, const double* hv1, double dt,
+                        double* prev, double* prev2, double* prev_energy, double* prev_dt, int32_t* instab,
+                        double* lte_out) {
+    const int32_t n = nl->num_nodes;
+    orc_arbiter a;
+    a.prev = prev; a.prev2 = prev2;
+    a.dvdt = (double*)xmalloc(sizeof(double) * (size_t)n);
+    a.prev_energy = *prev_energy; a.prev_dt = *prev_dt; a.instab = *instab; a.taa_init = 0;
+    if (lte_out) *lte_out = arb_lte(nl, &a, n, cur, hv0, hv1, dt);
+    const int r = arb_analyze(nl, &a, n, cur, hv0, hv1, dt);
+    *prev_energy = a.prev_energy; *prev_dt = a.prev_dt; *instab = a.instab;
+    free(a.dvdt);
+    return r;
+}
+
+/* Transient of one instance: the restated stepTransient
+ * (solvers/circuitsim_solver.cpp:559-780) driven as its caller drives it
+ * (engine_impl.cpp:205-256: every step asks for h; the loop runs while
+ * t < t_stop), or `steps` fixed steps of h. */
 int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_opts* opts, double* v_io,
-                        int32_t n_probe, const int32_t* probe, double* wave, orc_tran_out* out) {
+                        int32_t n_probe, const int32_t* probe, double* wave_t, double* wave, orc_tran_out* out) {
     orc_tran_opts o;
     if (opts) o = *opts; else orc_tran_opts_default(&o);
     memset(out, 0, sizeof(*out));
     const int32_t n = nl->num_nodes;
-    if (n <= 0 || !t || !t->transient || !(o.h > 0)) return NR_ERR_INVALID;
+    const int fixed = (o.control == ORC_TRAN_FIXED);
+    if (n <= 0 || !t || !t->transient || !(o.h >= 1e-18) || o.decimation < 1) return NR_ERR_INVALID;
+    if (fixed ? o.steps <= 0 : !(o.t_stop > 0)) return NR_ERR_INVALID;
+    if (o.method < ORC_INT_TRAP || o.method > ORC_INT_GEAR2) return NR_ERR_INVALID;
+    const int32_t cap = orc_wave_capacity(&o);
     orc_soa* s = orc_tensorize(nl);
-    double* v = (double*)xmalloc(sizeof(double) * (size_t)n);
-    double* vn = (double*)xmalloc(sizeof(double) * (size_t)n);
-    double* rhs = (double*)xmalloc(sizeof(double) * (size_t)n);
-    double* res = (double*)xmalloc(sizeof(double) * (size_t)n);
+    const size_t nb = sizeof(double) * (size_t)n;
+    double* v = (double*)xmalloc(nb);
+    double* vn = (double*)xmalloc(nb);
+    double* rhs = (double*)xmalloc(nb);
+    double* res = (double*)xmalloc(nb);
     double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
     double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
     double* vs = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_vsrc + 1));
-    double* vprev = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    double* is = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_isrc + 1));
+    double* hv0 = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    double* hv1 = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    double* hi0 = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
+    orc_arbiter arb;
+    arb.prev = (double*)xmalloc(nb); arb.prev2 = (double*)xmalloc(nb); arb.dvdt = (double*)xmalloc(nb);
     const int p32 = (o.precision == ORC_PREC_FP32_REFINE);
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
-    /* start state (D11): zero voltages and history, or the caller's state */
-    if (o.start) memcpy(v, v_io, sizeof(double) * (size_t)n);
-    else memset(v, 0, sizeof(double) * (size_t)n);
-    for (int32_t i = 0; i < nl->n_cap; ++i) vprev[i] = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
-    if (wave) {
-        for (int64_t k = 0; k < (int64_t)(o.steps + 1) * n_probe; ++k) wave[k] = NAN;
-        for (int32_t p = 0; p < n_probe; ++p) wave[p] = volt(v, n, probe[p]);
+    /* start state (D11): zero voltages, history and arbiter (the reference's
+     * IntegratorArbiter::initialize, :135-173, and zero-filled history), or
+     * the caller's state with the history and arbiter at it */
+    if (o.start) memcpy(v, v_io, nb);
+    else memset(v, 0, nb);
+    for (int32_t i = 0; i < nl->n_cap; ++i) {
+        hv0[i] = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
+        hv1[i] = hv0[i];
+        hi0[i] = 0.0;
     }
-    tran_ctx tc = {o.h, vs, vprev, 0};
-    int32_t total = 0, steps_done = 0, status = NR_STATUS_CONVERGED;
-    double kcl = 0.0;
-    for (int32_t step = 0; step < o.steps; ++step) {
-        const double tnext = (double)(step + 1) * o.h;
-        for (int32_t i = 0; i < nl->n_vsrc; ++i)
-            vs[i] = (nl->vsrc_is_pulse && nl->vsrc_is_pulse[i] && nl->vsrc_pulse)
-                        ? orc_pulse_value(nl->vsrc_pulse + 7 * (size_t)i, tnext) : nl->vsrc_volts[i];
-        int conv = 0;
-        int32_t it = 0;
-        for (; it < o.max_iter; ++it) {
-            int rc;
-            if (p32) {
-                rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, vals, av, a32, r32, x32, res, vn);
-            } else {
-                stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
-                orc_assemble(t, vals, av);
-                rc = (o.lu_kind == ORC_LU_DENSE)
-                         ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
-                         : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
-                                                        : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+    memcpy(arb.prev, v, nb); memcpy(arb.prev2, v, nb); memset(arb.dvdt, 0, nb);
+    arb.prev_energy = o.start ? cap_energy(nl, n, v) : 0.0;
+    arb.prev_dt = 1e-9;
+    arb.instab = 0; arb.taa_init = 0;
+    if (wave_t) for (int32_t k = 0; k < cap; ++k) wave_t[k] = NAN;
+    if (wave) for (int64_t k = 0; k < (int64_t)cap * n_probe; ++k) wave[k] = NAN;
+    int64_t rec = 0;
+    int32_t dec = 0;
+#define RECORD(time_)                                                                       \
+    do {                                                                                    \
+        const int32_t slot_ = (int32_t)(rec % cap);                                         \
+        if (wave_t) wave_t[slot_] = (time_);                                                \
+        if (wave) for (int32_t p_ = 0; p_ < n_probe; ++p_)                                  \
+            wave[(size_t)slot_ * n_probe + p_] = volt(v, n, probe[p_]);                     \
+        ++rec;                                                                              \
+    } while (0)
+    RECORD(0.0);
+    int32_t method = o.method, total = 0, steps_done = 0, rejected = 0, switched_at = -1, taa = 0;
+    int32_t status = NR_STATUS_CONVERGED;
+    double kcl = 0.0, tcur = 0.0, h_last = o.h;
+    while (fixed ? steps_done < o.steps : tcur < o.t_stop) {
+        double h = o.h;
+        int32_t attempt = 0, fail_status = NR_STATUS_NOT_CONVERGED;
+        int accepted = 0;
+        while (attempt < (fixed ? 1 : 4)) {
+            h_last = h;
+            /* Newton estimate: the arbiter's last voltages (getPrevVoltages +
+             * rejectHistory, :587-595); the fixed control continues from the
+             * last accepted step */
+            if (!fixed) {
+                memcpy(v, arb.prev, nb);
+                taa += arb_reject_history(&arb, n, v, h);
             }
-            if (rc != NR_OK) { status = NR_STATUS_SINGULAR; total += it + 1; kcl = 1e9; break; }
-            for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
-            kcl = kcl_soa(nl, s, vn, res, &tc);
-            int c = orc_check_convergence(n, vn, v, kcl, 1e-12);
-            memcpy(v, vn, sizeof(double) * (size_t)n);
-            if (c) { conv = 1; total += it + 1; break; }
+            const double tn = tcur + h;
+            for (int32_t i = 0; i < nl->n_vsrc; ++i)
+                vs[i] = orc_wave_value(nl->vsrc_wave ? nl->vsrc_wave[i] : 0,
+                                       nl->vsrc_wave_par ? nl->vsrc_wave_par + 7 * (size_t)i : NULL,
+                                       nl->vsrc_volts[i], tn);
+            for (int32_t i = 0; i < nl->n_isrc; ++i)
+                is[i] = orc_wave_value(nl->isrc_wave ? nl->isrc_wave[i] : 0,
+                                       nl->isrc_wave_par ? nl->isrc_wave_par + 7 * (size_t)i : NULL,
+                                       nl->isrc_amps[i], tn);
+            tran_ctx tc = {h, method, vs, is, hv0, hv1, hi0, 0};
+            int conv = 0, sing = 0;
+            int32_t it = 0;
+            for (; it < o.max_iter; ++it) {
+                int rc;
+                if (p32) {
+                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, vals, av, a32, r32, x32, res, vn);
+                } else {
+                    stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
+                    orc_assemble(t, vals, av);
+                    rc = (o.lu_kind == ORC_LU_DENSE)
+                             ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
+                             : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
+                                                            : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+                }
+                if (rc != NR_OK) { sing = 1; break; }
+                for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
+                kcl = kcl_soa(nl, s, vn, res, &tc);
+                int c = orc_check_convergence(n, vn, v, kcl, 1e-12);
+                memcpy(v, vn, nb);
+                if (c) { conv = 1; break; }
+            }
+            if (sing) { total += it + 1; kcl = 1e9; fail_status = NR_STATUS_SINGULAR; }
+            else if (!conv) { total += o.max_iter; fail_status = NR_STATUS_NOT_CONVERGED; }
+            else total += it + 1;
+            if (!conv) { ++rejected; h *= 0.5; ++attempt; continue; }
+            if (fixed) { accepted = 1; break; }
+            const int sug = arb_analyze(nl, &arb, n, v, hv0, hv1, h);
+            if (sug == 1) { ++rejected; h *= 0.5; ++attempt; fail_status = NR_STATUS_NOT_CONVERGED; continue; }
+            if (sug == 2 && method == ORC_INT_TRAP) {
+                ++rejected;
+                method = ORC_INT_GEAR2;
+                switched_at = steps_done;
+                continue;
+            }
+            accepted = 1;
+            break;
         }
-        if (status == NR_STATUS_SINGULAR) break;
-        if (!conv) { total += o.max_iter; status = NR_STATUS_NOT_CONVERGED; break; }
-        for (int32_t i = 0; i < nl->n_cap; ++i) vprev[i] = volt(v, n, nl->cap_n1[i]) - volt(v, n, nl->cap_n2[i]);
-        if (wave)
-            for (int32_t p = 0; p < n_probe; ++p) wave[(size_t)(step + 1) * n_probe + p] = volt(v, n, probe[p]);
+        if (!accepted) { status = fail_status; break; }
+        /* history of the accepted step, with the method in effect
+         * (solvers/circuitsim_solver.cpp:668-699) */
+        for (int32_t i = 0; i < nl->n_cap; ++i) {
+            double vc = 0.0;
+            if (nl->cap_n1[i] > 0 && nl->cap_n1[i] <= n) vc += v[nl->cap_n1[i] - 1];
+            if (nl->cap_n2[i] > 0 && nl->cap_n2[i] <= n) vc -= v[nl->cap_n2[i] - 1];
+            const double C = nl->cap_farads[i];
+            double ic;
+            if (method == ORC_INT_TRAP) ic = (2.0 * C / h) * (vc - hv0[i]) - hi0[i];
+            else if (method == ORC_INT_GEAR1) ic = (C / h) * (vc - hv0[i]);
+            else if (hv1[i] == 0.0 && tcur > h) ic = (C / h) * (vc - hv0[i]);
+            else ic = (C / h) * (1.5 * vc - 2.0 * hv0[i] + 0.5 * hv1[i]);
+            hv1[i] = hv0[i];
+            hv0[i] = vc;
+            hi0[i] = ic;
+        }
+        tcur = tcur + h;
         ++steps_done;
+        if (++dec >= o.decimation) { dec = 0; RECORD(tcur); }
     }
+#undef RECORD
     out->iterations = total;
     out->status = status;
     out->steps_done = steps_done;
+    out->rejected = rejected;
+    out->method = method;
+    out->switched_at = switched_at;
+    out->instability = arb.instab;
+    out->taa_rejections = taa;
+    out->recorded = rec;
     out->kcl = kcl;
-    memcpy(v_io, v, sizeof(double) * (size_t)n);
-    free(v); free(vn); free(rhs); free(res); free(vals); free(av); free(vs); free(vprev);
+    out->t_end = tcur;
+    out->h_last = h_last;
+    memcpy(v_io, v, nb);
+    free(v); free(vn); free(rhs); free(res); free(vals); free(av); free(vs); free(is);
+    free(hv0); free(hv1); free(hi0); free(arb.prev); free(arb.prev2); free(arb.dvdt);
     free(a32); free(r32); free(x32);
     orc_soa_free(s);
     return NR_OK;
@@ -1581,13 +1856,16 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
 
 int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
                               const orc_tran_opts* opts, int32_t threads, double* v_io, int32_t n_probe,
-                              const int32_t* probe, double* wave, orc_tran_out* outs) {
+                              const int32_t* probe, double* wave_t, double* wave, orc_tran_out* outs) {
     if (!t || B <= 0) return NR_ERR_INVALID;
     const int32_t n = nl->num_nodes;
-    const int32_t steps = opts ? opts->steps : 200;
+    orc_tran_opts o;
+    if (opts) o = *opts; else orc_tran_opts_default(&o);
+    const int32_t cap = orc_wave_capacity(&o);
+    int bad = 0;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
 #endif
     for (int32_t b = 0; b < B; ++b) {
         nr_netlist inst = *nl;
@@ -1600,10 +1878,11 @@ int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
         }
-        orc_solve_transient(&inst, t, opts, v_io + (size_t)b * n, n_probe, probe,
-                            wave ? wave + (size_t)b * (steps + 1) * n_probe : NULL, &outs[b]);
+        bad |= orc_solve_transient(&inst, t, &o, v_io + (size_t)b * n, n_probe, probe,
+                                   wave_t ? wave_t + (size_t)b * cap : NULL,
+                                   wave ? wave + (size_t)b * cap * n_probe : NULL, &outs[b]) != NR_OK;
     }
-    return NR_OK;
+    return bad ? NR_ERR_INVALID : NR_OK;
 }
 
 /* SoA arrays of tensorizeNetlist (same layout as the reference harness's

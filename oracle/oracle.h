@@ -120,32 +120,58 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
                        const double* const* fields, const orc_dc_opts* opts,
                        int32_t threads, double* v_out, orc_dc_out* outs);
 
-/* --- a16: fixed-step backward-Euler transient ----------------------------- */
+/* --- a16: transient (stepTransient restated) -------------------------------- */
 /* Pattern including the capacitor companion stamps (transient != 0). */
 int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** out);
 double orc_pulse_value(const double* p, double time);
+double orc_sine_value(const double* p, double time);
+/* value of a source with waveform kind NR_WAVE_* at `time` (dc: its DC value) */
+double orc_wave_value(int32_t kind, const double* par, double dc, double time);
+enum { ORC_INT_TRAP = 0, ORC_INT_GEAR1 = 1, ORC_INT_GEAR2 = 2 };   /* IntegrationType */
+enum { ORC_TRAN_FIXED = 0, ORC_TRAN_ARBITER = 1 };
 typedef struct orc_tran_opts {
-    double h;             /* step (s)                                     */
-    int32_t steps;
+    double h;             /* step (s); the arbiter's requested step       */
+    int32_t steps;        /* fixed control: steps to take                 */
     int32_t max_iter;     /* 50                                           */
     int32_t start;        /* 0: zero state (D11), 1: v_io holds the state */
     int32_t lu_kind;      /* ORC_LU_* (FP64 path)                         */
     int32_t precision;    /* ORC_PREC_*                                   */
+    int32_t method;       /* ORC_INT_* at t = 0                           */
+    int32_t control;      /* ORC_TRAN_*                                   */
+    double t_stop;        /* arbiter control: stop time (s)               */
+    int32_t wave_cap;     /* waveform ring entries (0: default, as in the C ABI) */
+    int32_t decimation;   /* record every k-th accepted step (>= 1)       */
 } orc_tran_opts;
 typedef struct orc_tran_out {
-    int32_t iterations;   /* NR iterations over all steps                 */
+    int32_t iterations;   /* NR iterations over all attempts              */
     int32_t status;       /* NR_STATUS_*                                  */
-    int32_t steps_done;
+    int32_t steps_done;   /* accepted steps                               */
+    int32_t rejected;     /* attempts not accepted                        */
+    int32_t method;       /* ORC_INT_* at the end                         */
+    int32_t switched_at;  /* accepted steps before the Gear-2 switch, or -1 */
+    int32_t instability;  /* arbiter instability counter at the end       */
+    int32_t taa_rejections; /* nodes whose predictor rejectHistory replaced */
+    int64_t recorded;     /* waveform entries written                     */
     double kcl;           /* KCL of the last Newton iterate (A)           */
+    double t_end;         /* time of the last accepted step               */
+    double h_last;        /* step of the last attempt                     */
 } orc_tran_out;
 void orc_tran_opts_default(orc_tran_opts* o);
-/* wave (optional): [(steps+1) * n_probe], row 0 = start state, NaN where
- * a step did not complete */
+/* resolved waveform ring capacity of `o` */
+int32_t orc_wave_capacity(const orc_tran_opts* o);
+/* wave_t (optional) [cap], wave (optional) [cap * n_probe]: the waveform
+ * ring, entry k in slot k mod cap, NaN where never written */
+/* IntegratorArbiter::analyzeStability on a given state (tests): 0 stable
+ * (prev / prev2 / prev_energy / prev_dt take the solution), 1 reduce the
+ * step, 2 switch the method; *instab in/out, *lte_out = calculateLTE */
+int orc_arbiter_analyze(const nr_netlist* nl, const double* cur, const double* hv0, const double* hv1, double dt,
+                        double* prev, double* prev2, double* prev_energy, double* prev_dt, int32_t* instab,
+                        double* lte_out);
 int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_opts* opts, double* v_io,
-                        int32_t n_probe, const int32_t* probe, double* wave, orc_tran_out* out);
+                        int32_t n_probe, const int32_t* probe, double* wave_t, double* wave, orc_tran_out* out);
 int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
                               const orc_tran_opts* opts, int32_t threads, double* v_io, int32_t n_probe,
-                              const int32_t* probe, double* wave, orc_tran_out* outs);
+                              const int32_t* probe, double* wave_t, double* wave, orc_tran_out* outs);
 
 int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
                          double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,

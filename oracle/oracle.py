@@ -27,11 +27,15 @@ class OrcDcOpts(C.Structure):
 
 class OrcTranOpts(C.Structure):
     _fields_ = [("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
-                ("lu_kind", C.c_int32), ("precision", C.c_int32)]
+                ("lu_kind", C.c_int32), ("precision", C.c_int32), ("method", C.c_int32), ("control", C.c_int32),
+                ("t_stop", C.c_double), ("wave_cap", C.c_int32), ("decimation", C.c_int32)]
 
 
 class OrcTranOut(C.Structure):
-    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int32), ("kcl", C.c_double)]
+    _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int32),
+                ("rejected", C.c_int32), ("method", C.c_int32), ("switched_at", C.c_int32),
+                ("instability", C.c_int32), ("taa_rejections", C.c_int32), ("recorded", C.c_int64),
+                ("kcl", C.c_double), ("t_end", C.c_double), ("h_last", C.c_double)]
 
 
 class OrcDcOut(C.Structure):
@@ -47,6 +51,9 @@ LU_MULTIFRONTAL = 2
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
 
+INT_TRAP, INT_GEAR1, INT_GEAR2 = 0, 1, 2
+TRAN_FIXED, TRAN_ARBITER = 0, 1
+
 
 def lib():
     global _LIB
@@ -59,9 +66,15 @@ def lib():
         L.orc_topology_build_ex.argtypes = [C.POINTER(_abi.NrNetlist), C.c_int32, C.POINTER(C.c_void_p)]
         L.orc_pulse_value.argtypes = [_f64p, C.c_double]
         L.orc_pulse_value.restype = C.c_double
+        L.orc_sine_value.argtypes = [_f64p, C.c_double]
+        L.orc_sine_value.restype = C.c_double
+        L.orc_wave_capacity.argtypes = [C.POINTER(OrcTranOpts)]
+        L.orc_arbiter_analyze.argtypes = [C.POINTER(_abi.NrNetlist), _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p,
+                                          _f64p, _f64p, C.POINTER(C.c_int32), _f64p]
+        L.orc_wave_capacity.restype = C.c_int32
         L.orc_solve_transient_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32,
                                                 C.POINTER(_f64p), C.POINTER(OrcTranOpts), C.c_int32, _f64p,
-                                                C.c_int32, _i32p, _f64p, C.POINTER(OrcTranOut)]
+                                                C.c_int32, _i32p, _f64p, _f64p, C.POINTER(OrcTranOut)]
         L.orc_topology_free.argtypes = [C.c_void_p]
         L.orc_topology_info.argtypes = [C.c_void_p, _i32p]
         L.orc_topology_export.argtypes = [C.c_void_p, C.c_int32, _i32p, C.c_int64]
@@ -179,10 +192,33 @@ def pulse_value(p, t):
     return lib().orc_pulse_value(_p64(p), t)
 
 
+def sine_value(p, t):
+    p = np.ascontiguousarray(p, np.float64)
+    return lib().orc_sine_value(_p64(p), t)
+
+
+def arbiter_analyze(nl, cur, hv0, hv1, dt, prev, prev2, prev_energy, prev_dt, instab):
+    """IntegratorArbiter::analyzeStability on a given state; returns
+    (suggestion, lte, state) with state = dict(prev, prev2, prev_energy,
+    prev_dt, instab) after the call."""
+    c, keep = nl.to_c()
+    cur, hv0, hv1 = (np.ascontiguousarray(x, np.float64) for x in (cur, hv0, hv1))
+    prev, prev2 = np.array(prev, np.float64), np.array(prev2, np.float64)
+    pe, pdt, ins, lte = C.c_double(prev_energy), C.c_double(prev_dt), C.c_int32(instab), C.c_double(0.0)
+    r = lib().orc_arbiter_analyze(C.byref(c), _p64(cur), _p64(hv0) if len(hv0) else None,
+                                  _p64(hv1) if len(hv1) else None, dt, _p64(prev), _p64(prev2), C.byref(pe),
+                                  C.byref(pdt), C.byref(ins), C.byref(lte))
+    return r, lte.value, {"prev": prev, "prev2": prev2, "prev_energy": pe.value, "prev_dt": pdt.value,
+                          "instab": ins.value}
+
+
 def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, steps=200, max_iter=50, start=0,
-                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0, precision=PREC_FP64):
-    """Backward-Euler transient of B instances; returns (v_final [B, n],
-    wave [B, steps+1, n_probe], stats)."""
+                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0, precision=PREC_FP64,
+                          method=INT_GEAR1, control=TRAN_FIXED, t_stop=0.0, wave_capacity=0, decimation=1,
+                          full=False):
+    """Transient of B instances; returns (v_final [B, n], wave [B, cap, n_probe],
+    stats) and, with full=True, also the ring's times [B, cap].  The ring
+    holds entry k in slot k mod cap (entry 0 = the start state)."""
     c, keep = nl.to_c()
     arrs = (_f64p * _abi.N_FIELDS)()
     hold = []
@@ -195,13 +231,20 @@ def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, step
             arrs[fid] = None
     v = np.zeros((B, nl.num_nodes)) if v0 is None else np.array(v0, dtype=np.float64).reshape(B, nl.num_nodes)
     pr = np.ascontiguousarray(probes if probes is not None else [], np.int32)
-    wave = np.zeros((B, steps + 1, len(pr)))
+    o = OrcTranOpts(h, steps, max_iter, start, lu, precision, method, control, t_stop, wave_capacity, decimation)
+    cap = lib().orc_wave_capacity(C.byref(o))
+    wave = np.zeros((B, cap, len(pr)))
+    times = np.zeros((B, cap))
     outs = (OrcTranOut * B)()
-    o = OrcTranOpts(h, steps, max_iter, start, lu, precision)
-    lib().orc_solve_transient_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), len(pr),
-                                    _p32(pr) if len(pr) else None, _p64(wave), outs)
-    stats = {"iterations": np.array([x.iterations for x in outs]), "status": np.array([x.status for x in outs]),
-             "steps_done": np.array([x.steps_done for x in outs]), "kcl": np.array([x.kcl for x in outs])}
+    rc = lib().orc_solve_transient_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), len(pr),
+                                         _p32(pr) if len(pr) else None, _p64(times), _p64(wave), outs)
+    if rc != 0:
+        raise ValueError(f"orc_solve_transient_batch: invalid arguments (status {rc})")
+    stats = {k: np.array([getattr(x, k) for x in outs])
+             for k in ("iterations", "status", "steps_done", "rejected", "method", "switched_at", "instability",
+                       "taa_rejections", "recorded", "kcl", "t_end", "h_last")}
+    if full:
+        return v, wave, stats, times
     return v, wave, stats
 
 

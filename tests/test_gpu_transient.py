@@ -59,8 +59,8 @@ def test_diode_mesh_with_capacitors(ctx, O):
     nl.cap_n1, nl.cap_n2 = _i32(np.arange(1, n + 1)), _i32(np.zeros(n))
     nl.cap_farads = _f64(np.full(n, 2e-12))
     nl.vsrc_pos, nl.vsrc_neg, nl.vsrc_volts = _i32([1]), _i32([0]), _f64([0.0])
-    nl.vsrc_is_pulse = _i32([1])
-    nl.vsrc_pulse = _f64([[0.0, 2.0, 2e-9, 1e-9, 1e-9, 5e-9, 0.0]])
+    nl.vsrc_wave = _i32([_abi.NR_WAVE_PULSE])
+    nl.vsrc_wave_par = _f64([[0.0, 2.0, 2e-9, 1e-9, 1e-9, 5e-9, 0.0]])
     B, steps, h = 6, 30, 0.5e-9
     f = netlists.mc_fields(nl, B, seed=21)
     f[_abi.NR_FIELD_CAP_FARADS] = np.random.default_rng(3).uniform(1e-12, 3e-12, (B, n))

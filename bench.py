@@ -233,7 +233,8 @@ def logic_workload(a):
     """Random CMOS logic (levelised inverter DAG with switching PULSE
     inputs, level-1 MOSFETs), 2048 Monte-Carlo instances (VTO sigma 30 mV,
     KP sigma 3 %): DC operating points, then a fixed-step backward-Euler
-    transient from them.  The default size (2000 nodes) is the largest at
+    transient from them and the reference's trapezoidal transient with the
+    arbiter's step control over the same span.  The default size (2000 nodes) is the largest at
     which the reference's Newton method converges on this family."""
     from karam_amd import _abi, netlists, nr
     B = a.instances if a.instances != 4096 else 2048
@@ -269,6 +270,16 @@ def logic_workload(a):
                       probes=[2, 3, T.info["n"] // 2, T.info["n"]])
     t_tran = time.perf_counter() - t0
     stt = tr.stats()
+    # the reference's own transient: trapezoidal with the arbiter's step
+    # control over the same time span, from the same operating points
+    t_stop = a.tran_steps * 0.5e-9
+    t0 = time.perf_counter()
+    ra = tr.transient(h=0.5e-9, start=_abi.NR_TRAN_START_SEED, method=_abi.NR_INT_TRAP,
+                      control=_abi.NR_TRAN_ARBITER, t_stop=t_stop, probes=[2, 3, T.info["n"] // 2, T.info["n"]])
+    t_arb = time.perf_counter() - t0
+    sta, tsa = tr.stats(), tr.tran_stats()
+    log(f"arbiter transient: {t_arb:.2f} s, completed {ra['n_converged']}/{B}, accepted {tsa['accepted'].sum()}, "
+        f"rejected {tsa['rejected'].sum()}, switched {(tsa['switched_at'] >= 0).sum()}")
     out = {
         "metric": METRIC, "value": conv / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": total * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
@@ -285,6 +296,16 @@ def logic_workload(a):
                       "instance_steps_per_s": rt["n_converged"] * a.tran_steps / t_tran,
                       "newton_iterations_per_step": float(stt["iterations"].mean() / a.tran_steps),
                       "batched_sweeps": rt["batched_iterations"]},
+        "transient_arbiter": {"t_stop_s": t_stop, "h_s": 0.5e-9, "method": "trapezoidal, switched to Gear-2 "
+                              "by the arbiter", "start": "DC operating point",
+                              "instances_completed": int(ra["n_converged"]), "wall_s": t_arb,
+                              "accepted_steps": int(tsa["accepted"].sum()),
+                              "rejected_attempts": int(tsa["rejected"].sum()),
+                              "instances_switched": int((tsa["switched_at"] >= 0).sum()),
+                              "accepted_steps_per_s": float(tsa["accepted"].sum() / t_arb),
+                              "newton_iterations_per_accepted_step":
+                                  float(sta["iterations"].sum() / max(1, tsa["accepted"].sum())),
+                              "rounds_batched_sweeps": ra["batched_iterations"]},
         "roofline": None, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

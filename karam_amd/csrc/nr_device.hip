@@ -141,8 +141,9 @@ struct TranState {
     int method, attempt, instab, accepted, rejected, switched_at, glin_method, dec;
 };
 // InstState.done in a transient: 0 solving, 1 finished, 2 the attempt's
-// Newton solve converged, 3 it failed (the step control decides)
-enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3 };
+// Newton solve converged, 3 it failed (the step control decides), 4 the
+// next attempt is armed (k_tran_begin sets it up)
+enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3, DONE_ARMED = 4 };
 
 struct DevBatch {
     int B, Bp;
@@ -663,7 +664,8 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         InstState s;
         s.step = use_seed ? seeded_step : 0;
         s.it = 0; s.total = 0; s.ref = 0;
-        s.status = NR_STATUS_NOT_CONVERGED; s.done = 0; s.singular = 0; s.seeded = use_seed;
+        s.status = NR_STATUS_NOT_CONVERGED; s.done = mode != 0 ? DONE_ARMED : DONE_RUNNING; s.singular = 0;
+        s.seeded = use_seed;
         s.prev_kcl = 0.0; s.kcl = 0.0;
         P.st[b] = s;
         P.act[b] = b;
@@ -671,15 +673,19 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
     }
 }
 
-// Transient attempt set-up (one block per running instance): source values
-// at t + h (PULSE / SINE sources from their waveform, the others at their
-// DC value), the linear part of A when the step or the method changed, the
-// interleaved copy of the Newton estimate, and the linear rhs with the
-// capacitor history (FP32 path: the KCL vector at the estimate).
-__global__ __launch_bounds__(256) void k_tran_begin(DevTopo T, DevBatch P) {
+// Transient attempt set-up (one block per armed instance of the active
+// list): source values at t + h (PULSE / SINE sources from their waveform,
+// the others at their DC value), the linear part of A when the step or the
+// method changed, the interleaved copy of the Newton estimate, and the
+// linear rhs with the capacitor history (FP32 path: the KCL vector at the
+// estimate).  The instance then runs.
+__global__ __launch_bounds__(256) void k_tran_begin(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                    int nact_grid) {
 #pragma clang fp contract(off)
-    const int b = blockIdx.x;
-    if (P.st[b].done != DONE_RUNNING) return;
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    if ((int)blockIdx.x >= nact) return;
+    const int b = act[blockIdx.x];
+    if (P.st[b].done != DONE_ARMED) return;
     const TranState ts = P.ts[b];
     const double tn = ts.t + ts.h;
     for (int i = threadIdx.x; i < T.nV; i += blockDim.x)
@@ -701,6 +707,7 @@ __global__ __launch_bounds__(256) void k_tran_begin(DevTopo T, DevBatch P) {
     if (P.p32) residual_to_interleaved(T, P, b, v);
     else
         for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
+    if (threadIdx.x == 0) P.st[b].done = DONE_RUNNING;
 }
 
 __device__ __forceinline__ double lte_norm(double v) {
@@ -767,13 +774,28 @@ __device__ double block_lte(const DevTopo& T, const DevBatch& P, int b, const do
 //     (:668-699), t += h, waveform ring entry every `decimation` steps, the
 //     next step starts at h from the arbiter's voltages, or the instance is
 //     finished (`steps` steps, or t >= t_stop).
-__global__ __launch_bounds__(256) void k_tran_control(DevTopo T, DevBatch P, int control, int steps, double h0,
-                                                      double t_stop, int decimation, int n_probe,
-                                                      const int* __restrict__ probe, int wave_cap,
-                                                      double* __restrict__ wave_t, double* __restrict__ wave) {
+// Runs inside every transient sweep over the active list, so each instance
+// goes on to its next attempt without waiting for the others.
+struct TranArgs {
+    int control, steps, decimation, n_probe, wave_cap;
+    double h0, t_stop;
+    const int* probe;
+    double *wave_t, *wave;
+};
+
+__global__ __launch_bounds__(256) void k_tran_control(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                      int nact_grid, TranArgs A) {
 #pragma clang fp contract(off)
     __shared__ double red[8];
-    const int b = blockIdx.x;
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    if ((int)blockIdx.x >= nact) return;
+    const int b = act[blockIdx.x];
+    const int control = A.control, steps = A.steps, decimation = A.decimation, n_probe = A.n_probe;
+    const int wave_cap = A.wave_cap;
+    const double h0 = A.h0, t_stop = A.t_stop;
+    const int* __restrict__ probe = A.probe;
+    double* __restrict__ wave_t = A.wave_t;
+    double* __restrict__ wave = A.wave;
     const int n = T.n;
     InstState st = P.st[b];
     if (st.done != DONE_CONVERGED && st.done != DONE_FAILED) return;
@@ -887,12 +909,11 @@ __global__ __launch_bounds__(256) void k_tran_control(DevTopo T, DevBatch P, int
             st.done = DONE_FINISHED;
             st.status = NR_STATUS_CONVERGED;
         } else {
-            if (decision == RETRY && st.done == DONE_CONVERGED) st.status = NR_STATUS_NOT_CONVERGED;
-            st.done = DONE_RUNNING;
+            st.done = DONE_ARMED;
             st.it = 0;
             st.singular = 0;
             st.prev_kcl = 0.0;
-            if (decision == ACCEPT) st.status = NR_STATUS_NOT_CONVERGED;
+            st.status = NR_STATUS_NOT_CONVERGED;
         }
         P.st[b] = st;
         P.ts[b] = ts;

@@ -267,7 +267,7 @@ def logic_workload(a):
     tr.set_seed(V)
     t0 = time.perf_counter()
     rt = tr.transient(h=0.5e-9, steps=a.tran_steps, start=_abi.NR_TRAN_START_SEED,
-                      probes=[2, 3, T.info["n"] // 2, T.info["n"]])
+                      probes=[2, 3, T.info["n"] // 2, T.info["n"]], use_graph=a.tran_graph)
     t_tran = time.perf_counter() - t0
     stt = tr.stats()
     # the reference's own transient: trapezoidal with the arbiter's step
@@ -275,9 +275,11 @@ def logic_workload(a):
     t_stop = a.tran_steps * 0.5e-9
     t0 = time.perf_counter()
     ra = tr.transient(h=0.5e-9, start=_abi.NR_TRAN_START_SEED, method=_abi.NR_INT_TRAP,
-                      control=_abi.NR_TRAN_ARBITER, t_stop=t_stop, probes=[2, 3, T.info["n"] // 2, T.info["n"]])
+                      control=_abi.NR_TRAN_ARBITER, t_stop=t_stop, probes=[2, 3, T.info["n"] // 2, T.info["n"]],
+                      use_graph=a.tran_graph)
     t_arb = time.perf_counter() - t0
     sta, tsa = tr.stats(), tr.tran_stats()
+    Wa, Ta = tr.waveform(), tr.waveform_times()
     log(f"arbiter transient: {t_arb:.2f} s, completed {ra['n_converged']}/{B}, accepted {tsa['accepted'].sum()}, "
         f"rejected {tsa['rejected'].sum()}, switched {(tsa['switched_at'] >= 0).sum()}")
     out = {
@@ -308,6 +310,30 @@ def logic_workload(a):
                               "rounds_batched_sweeps": ra["batched_iterations"]},
         "roofline": None, "cpu_baseline": None,
     }
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        base, parity = cpu_baseline(nl, f, B, V, it, a.cpu_seconds, threads)
+        out["cpu_baseline"] = base
+        out["parity_sample"] = parity
+        # the arbiter transient of the first `threads` instances on the host
+        m = min(B, threads)
+        t0 = time.perf_counter()
+        _, Wo, so, To = O.solve_transient_batch(
+            nl, O.Topology(nl, transient=True), {k: v[:m] for k, v in f.items()}, m, h=0.5e-9, start=1, v0=V[:m],
+            method=O.INT_TRAP, control=O.TRAN_ARBITER, t_stop=t_stop, threads=threads,
+            probes=[2, 3, T.info["n"] // 2, T.info["n"]], full=True)
+        t_cpu = time.perf_counter() - t0
+        same = [bool(np.array_equal(Wa[b], Wo[b]) and np.array_equal(Ta[b], To[b]) and
+                     tsa["accepted"][b] == so["steps_done"][b] and sta["iterations"][b] == so["iterations"][b])
+                for b in range(m)]
+        out["transient_arbiter"]["cpu_baseline"] = {
+            "value": float(so["steps_done"].sum() / t_cpu), "unit": "accepted steps/s", "cores": threads,
+            "kind": "port", "sample": f"first {m} of {B} instances over the same span, FP64 oracle, one instance "
+                                      f"per OpenMP thread"}
+        out["transient_arbiter"]["parity_sample"] = {"instances": m, "bit_identical": int(sum(same))}
+        log(f"cpu baseline {base['value']:.1f} solves/s; arbiter transient {so['steps_done'].sum() / t_cpu:.1f} "
+            f"accepted steps/s on {threads} threads; bit-identical {sum(same)}/{m}")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -416,6 +442,7 @@ def main():
     ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--logic-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
+    ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     a = ap.parse_args()

@@ -116,6 +116,45 @@ typedef struct nr_netlist {
                                     names a model with no card               */
 } nr_netlist;
 
+/* ---- SPICE netlist ingestion -------------------------------------------- */
+/* SPICEParser::parse (netlist/spice_parser.cpp:148-261, the entry the
+ * session's deserialiseRequest calls, engine_impl.cpp:84-95): SPICE text to
+ * an nr_netlist with the reference parser's node numbering (first
+ * appearance; 0, GND, GROUND and VSS are ground) and component order.
+ * Filled in where that parser leaves the netlist undefined or drops what the
+ * line says: diode Is / N / Vt from the diode's .model card (defaults 1e-14
+ * A, 1, 0.02585 V); a PULSE / SIN source without a DC value takes its t = 0
+ * value; SIN / SINE select NR_WAVE_SINE; MOSFET W / L are read; AC on a
+ * current source is skipped.  Elements the batched Newton path does not
+ * simulate (L, K, Q, J, E, G, H, F, T, .subckt / X) make the parse fail with
+ * NR_ERR_UNSUPPORTED and the line number in nr_last_error(NULL).  Unknown
+ * element letters are skipped with a warning, as the reference does. */
+typedef struct nr_spice nr_spice;
+typedef struct nr_spice_info {
+    int32_t lines;           /* lines after the title (SPICEParseResult)    */
+    int32_t elements;        /* element lines, skipped ones included        */
+    int32_t models;          /* .model cards                                */
+    int32_t nodes;           /* numGlobalNodes                              */
+    int32_t has_op, has_tran, has_ac, has_dc;
+    double tran_step, tran_stop, tran_start;
+    int32_t warnings;
+    int32_t pad;
+} nr_spice_info;
+enum { NR_SPICE_VSS_IS_NET = 1 };   /* flag: VSS is an ordinary net         */
+NR_API int nr_spice_parse(const char* text, int64_t len, int32_t flags, nr_spice** out);
+NR_API int nr_spice_free(nr_spice* p);
+/* A view of the parsed netlist: its arrays stay valid until nr_spice_free. */
+NR_API int nr_spice_get_netlist(const nr_spice* p, nr_netlist* out);
+NR_API int nr_spice_get_info(const nr_spice* p, nr_spice_info* out);
+/* Names, NUL-terminated into buf (cap bytes): node k (upper-cased, as the
+ * parser keys it); element `index` of kind 'R' 'C' 'V' 'I' 'D' 'M' (its
+ * first token); warning k. */
+NR_API int nr_spice_node_name(const nr_spice* p, int32_t node, char* buf, int32_t cap);
+NR_API int nr_spice_element_name(const nr_spice* p, char kind, int32_t index, char* buf, int32_t cap);
+NR_API int nr_spice_warning(const nr_spice* p, int32_t k, char* buf, int32_t cap);
+/* AC magnitude / phase of voltage source i (read, not simulated here). */
+NR_API int nr_spice_vsrc_ac(const nr_spice* p, int32_t i, double* mag, double* phase_deg);
+
 /* ---- per-instance parameter fields (Monte-Carlo / corners) -------------- */
 /* nr_batch_set_params(field, values[B * count]) — row-major per instance,
  * netlist order, absolute values (already perturbed). */

@@ -58,6 +58,7 @@ NR_INT_GEAR1 = 1
 NR_INT_GEAR2 = 2
 NR_TRAN_FIXED = 0
 NR_TRAN_ARBITER = 1
+NR_SPICE_VSS_IS_NET = 1
 
 _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
@@ -174,4 +175,13 @@ class NrTiming(C.Structure):
     _fields_ = [
         ("ms", C.c_double * NR_PHASE_COUNT), ("launches", C.c_int64 * NR_PHASE_COUNT),
         ("factor_kernel_ms", C.c_double), ("factor_instances", C.c_int64),
+    ]
+
+
+class NrSpiceInfo(C.Structure):
+    _fields_ = [
+        ("lines", C.c_int32), ("elements", C.c_int32), ("models", C.c_int32), ("nodes", C.c_int32),
+        ("has_op", C.c_int32), ("has_tran", C.c_int32), ("has_ac", C.c_int32), ("has_dc", C.c_int32),
+        ("tran_step", C.c_double), ("tran_stop", C.c_double), ("tran_start", C.c_double),
+        ("warnings", C.c_int32), ("pad", C.c_int32),
     ]

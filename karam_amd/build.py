@@ -3,7 +3,7 @@
     python -m karam_amd.build [--verbose]
 
 The library is one HIP translation unit (kernels + host driver + C ABI)
-plus the host-only topology builder.  Packed-FP32 instruction selection is
+plus the host-only topology builder and SPICE parser.  Packed-FP32 instruction selection is
 disabled for all device code.
 """
 from __future__ import annotations
@@ -16,8 +16,8 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 LIB = os.path.join(PKG, "libkaram_nr.so")
-SOURCES = [os.path.join(PKG, "csrc", "nr_device.hip"), os.path.join(PKG, "csrc", "topology.cpp")]
-DEPENDS = SOURCES + [os.path.join(PKG, "csrc", f) for f in ("nr_host.inc", "seeder.inc", "topology.h")] + \
+SOURCES = [os.path.join(PKG, "csrc", f) for f in ("nr_device.hip", "topology.cpp", "spice.cpp")]
+DEPENDS = SOURCES + [os.path.join(PKG, "csrc", f) for f in ("nr_host.inc", "seeder.inc", "topology.h", "spice.h")] + \
     [os.path.join(ROOT, "include", "karam_nr.h")]
 
 

@@ -49,6 +49,7 @@
 #include <vector>
 
 #include "karam_nr.h"
+#include "spice.h"
 #include "topology.h"
 
 using karam::Topology;

@@ -44,6 +44,8 @@ EXPORTED = [
     "nr_seeder_create", "nr_seeder_destroy", "nr_seeder_record", "nr_seeder_history", "nr_seeder_reset",
     "nr_train_opts_default", "nr_seeder_train", "nr_seeder_predict", "nr_batch_set_seed_from",
     "nr_seeder_param_count", "nr_seeder_get_params", "nr_seeder_set_params", "nr_seeder_loss_grad",
+    "nr_spice_parse", "nr_spice_free", "nr_spice_get_netlist", "nr_spice_get_info", "nr_spice_node_name",
+    "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
 ]
 
 
@@ -70,6 +72,14 @@ def lib():
         L.nr_batch_autotune.argtypes = [_vp, C.c_int32, C.c_int32, C.POINTER(_abi.NrAutotuneResult)]
         L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_waveform_times.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_spice_parse.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.POINTER(_vp)]
+        L.nr_spice_free.argtypes = [_vp]
+        L.nr_spice_get_netlist.argtypes = [_vp, C.POINTER(_abi.NrNetlist)]
+        L.nr_spice_get_info.argtypes = [_vp, C.POINTER(_abi.NrSpiceInfo)]
+        L.nr_spice_node_name.argtypes = [_vp, C.c_int32, C.c_char_p, C.c_int32]
+        L.nr_spice_element_name.argtypes = [_vp, C.c_char, C.c_int32, C.c_char_p, C.c_int32]
+        L.nr_spice_warning.argtypes = [_vp, C.c_int32, C.c_char_p, C.c_int32]
+        L.nr_spice_vsrc_ac.argtypes = [_vp, C.c_int32, _f64p, _f64p]
         L.nr_batch_get_wave_recorded.argtypes = [_vp, C.POINTER(C.c_int64), C.c_int64]
         L.nr_batch_get_tran_stats.argtypes = [_vp, C.POINTER(_abi.NrTranStats), C.c_int64]
         _f32p = C.POINTER(C.c_float)

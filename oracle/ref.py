@@ -47,6 +47,8 @@ def lib():
         L.ref_vcrit.restype = C.c_double
         L.ref_diode_i_clamped.argtypes = [C.c_double] * 4
         L.ref_diode_i_clamped.restype = C.c_double
+        L.ref_spice_dump.argtypes = [C.c_char_p, C.c_char_p, C.c_int64]
+        L.ref_spice_dump.restype = C.c_int64
         _LIB = L
     return _LIB
 
@@ -159,3 +161,12 @@ def vcrit(vt, Is):
 
 def diode_i_clamped(vd, Is, N, Vt):
     return lib().ref_diode_i_clamped(vd, Is, N, Vt)
+
+
+def spice_dump(text: str) -> str:
+    """What SPICEParser::parse makes of `text`, as ref_spice_dump's lines."""
+    raw = text.encode()
+    need = lib().ref_spice_dump(raw, None, 0)
+    buf = C.create_string_buffer(int(need) + 1)
+    lib().ref_spice_dump(raw, buf, need + 1)
+    return buf.value.decode()

@@ -1,5 +1,6 @@
 // ref_harness.cpp — thin C entry points over the reference engine's own
 // compilable translation units (math/sparse_lu.cpp, math/linalg.h,
+// netlist/spice_parser.cpp,
 // infrastructure/factor_graph.cpp, infrastructure/treewidth_analyzer.cpp,
 // tensors/physics_tensors.cpp, tensors/kernel_dispatch.cpp,
 // physics/device_physics.h), compiled from /root/reference by oracle/Makefile
@@ -8,6 +9,7 @@
 // TEST INFRASTRUCTURE ONLY: it pins the oracle restatement against the
 // reference code in the build container.  Nothing in the product links it.
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <string>
@@ -18,6 +20,7 @@
 #include "math/linalg.h"
 #include "math/sparse_lu.h"
 #include "netlist/circuit.h"
+#include "netlist/spice_parser.h"
 #include "physics/device_physics.h"
 #include "tensors/kernel_dispatch.h"
 #include "tensors/physics_tensors.h"
@@ -187,6 +190,65 @@ double ref_vcrit(double vt, double is) { return compute_vcrit(vt, is); }
 double ref_diode_i_clamped(double vd, double Is, double N, double Vt) {
     DiodeParams<double> p = {Is, N, Vt};
     return diode_current_clamped(vd, p, 30.0);
+}
+
+// SPICEParser::parse (netlist/spice_parser.cpp:148-261) of `text`, dumped
+// as lines of text into out (cap bytes); returns the length needed.  Fields
+// the parser leaves uninitialised are not read: a source's DC value only
+// when its type is "DC", never a diode's Is / N / Vt.
+int64_t ref_spice_dump(const char* text, char* out, int64_t cap) {
+    acutesim::SPICEParser parser;
+    acutesim::SPICEParseResult r = parser.parse(std::string(text));
+    std::string o;
+    char buf[512];
+    auto put = [&](const char* fmt, auto... a) {
+        std::snprintf(buf, sizeof(buf), fmt, a...);
+        o += buf;
+    };
+    put("success %d\nerror_line %d\nlines %d\nelements %d\nmodels %d\nnodes %d\n", (int)r.success, r.errorLine,
+        r.linesProcessed, r.elementsCreated, r.modelsCreated, r.netlist.numGlobalNodes);
+    put("directives %d %d %d %d\ntran %.17g %.17g %.17g\n", (int)r.hasOP, (int)r.hasTRAN, (int)r.hasAC,
+        (int)r.hasDC, r.tranStep, r.tranStop, r.tranStart);
+    const TensorBlock& g = r.netlist.globalBlock;
+    for (const auto& x : g.resistors)
+        put("R %s %d %d %.17g\n", x.name.c_str(), x.nodeTerminal1, x.nodeTerminal2, x.resistance_ohms);
+    for (const auto& x : g.capacitors)
+        put("C %s %d %d %.17g\n", x.name.c_str(), x.nodePlate1, x.nodePlate2, x.capacitance_farads);
+    for (const auto& x : g.voltageSources) {
+        const bool dc = x.type == "DC";
+        put("V %d %d %s %d %.17g %.17g %.17g", x.nodePositive, x.nodeNegative, x.type.empty() ? "-" : x.type.c_str(),
+            (int)dc, dc ? x.voltage_V : 0.0, x.ac_mag, x.ac_phase);
+        for (double q : {x.pulse_v1, x.pulse_v2, x.pulse_td, x.pulse_tr, x.pulse_tf, x.pulse_pw, x.pulse_per,
+                         x.sine_vo, x.sine_va, x.sine_freq, x.sine_td, x.sine_theta, x.sine_phase})
+            put(" %.17g", q);
+        o += "\n";
+    }
+    for (const auto& x : g.currentSources) {
+        const bool dc = x.type == "DC";
+        put("I %d %d %s %d %.17g", x.nodePositive, x.nodeNegative, x.type.empty() ? "-" : x.type.c_str(), (int)dc,
+            dc ? x.current_A : 0.0);
+        for (double q : {x.pulse_v1, x.pulse_v2, x.pulse_td, x.pulse_tr, x.pulse_tf, x.pulse_pw, x.pulse_per,
+                         x.sine_vo, x.sine_va, x.sine_freq, x.sine_td, x.sine_theta, x.sine_phase})
+            put(" %.17g", q);
+        o += "\n";
+    }
+    for (const auto& x : g.diodes)
+        put("D %s %d %d %s\n", x.instanceName.c_str(), x.anode, x.cathode, x.modelName.c_str());
+    for (const auto& x : g.mosfets)
+        put("M %s %d %d %d %d %s %.17g %.17g\n", x.instanceName.c_str(), x.drain, x.gate, x.source, x.body,
+            x.modelName.c_str(), x.w, x.l);
+    for (const auto& kv : r.netlist.modelCards) {
+        put("MODEL %s %s %d", kv.second.name.c_str(), kv.second.type.c_str(), kv.second.level);
+        for (const auto& p : kv.second.params) put(" %s=%.17g", p.first.c_str(), p.second);
+        o += "\n";
+    }
+    put("counts L %d K %d Q %d J %d E %d G %d H %d F %d T %d X %d S %d\n", (int)g.inductors.size(),
+        (int)g.mutualInductors.size(), (int)g.bjts.size(), (int)g.jfets.size(),
+        (int)g.voltageControlledVoltageSources.size(), (int)g.voltageControlledCurrentSources.size(),
+        (int)g.currentControlledVoltageSources.size(), (int)g.currentControlledCurrentSources.size(),
+        (int)g.transmissionLines.size(), (int)r.netlist.instances.size(), r.subcircuitsCreated);
+    if ((int64_t)o.size() < cap) std::memcpy(out, o.c_str(), o.size() + 1);
+    return (int64_t)o.size();
 }
 
 }  // extern "C"

@@ -1,0 +1,209 @@
+"""SPICE ingestion (nr_spice_parse) against the reference's own parser.
+
+tests/golden/spice/<name>.ref.txt is what SPICEParser::parse
+(netlist/spice_parser.cpp, compiled from the reference tree by
+tests/golden/make_spice_golden.py) makes of <name>.sp: node ids, element order
+and values, waveform parameters, model cards and directives.  The library's
+parser must give the same, except where it fills in what the reference
+leaves undefined or drops (diode Is / N / Vt, the DC value of waveform-only
+sources, SIN typed so the stamps evaluate it, MOSFET W / L, AC on current
+sources), which is checked against the rule instead.  Host only."""
+from __future__ import annotations
+
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from karam_amd import _abi, netlists
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "spice")
+CASES = sorted(os.path.basename(p)[:-3] for p in glob.glob(os.path.join(HERE, "*.sp")) if "unsupported" not in p)
+
+
+def read_ref(name):
+    ref = {"R": [], "C": [], "V": [], "I": [], "D": [], "M": [], "MODEL": []}
+    with open(os.path.join(HERE, name + ".ref.txt")) as f:
+        for line in f:
+            t = line.split()
+            k = t[0]
+            if k in ref:
+                ref[k].append(t[1:])
+            elif k == "directives":
+                ref[k] = [int(x) for x in t[1:]]
+            elif k == "tran":
+                ref[k] = [float(x) for x in t[1:]]
+            elif k == "counts":
+                ref[k] = t[1:]
+            else:
+                ref[k] = int(t[1])
+    return ref
+
+
+def parse(name):
+    from karam_amd import spice
+    with open(os.path.join(HERE, name + ".sp")) as f:
+        return spice.parse(f.read())
+
+
+# sources whose line carries "AC" on a current source: the reference reads
+# its magnitude as the DC value (checked in test_fills_in_...)
+AC_ON_CURRENT_SOURCE = {("oddities", 0)}
+# waveform sources with an explicit DC value ("DC 1.2 PULSE(...)"): the
+# dump reports their DC value only for type "DC"
+EXPLICIT_DC = {("waves", "V", 1): 1.2}
+WAVE = {"PULSE": _abi.NR_WAVE_PULSE, "SIN": _abi.NR_WAVE_SINE, "DC": _abi.NR_WAVE_DC, "-": _abi.NR_WAVE_DC}
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_matches_reference_parser(name):
+    nl, info = parse(name)
+    ref = read_ref(name)
+    assert ref["success"] == 1
+    assert nl.num_nodes == ref["nodes"] == info["nodes"]
+    assert (info["lines"], info["elements"], info["models"]) == (ref["lines"], ref["elements"], ref["models"])
+    assert [info[k] for k in ("has_op", "has_tran", "has_ac", "has_dc")] == ref["directives"]
+    assert [info["tran_step"], info["tran_stop"], info["tran_start"]] == ref["tran"]
+    # resistors and capacitors: names, nodes, values
+    for kind, n1, n2, val in (("R", nl.res_n1, nl.res_n2, nl.res_ohms), ("C", nl.cap_n1, nl.cap_n2, nl.cap_farads)):
+        r = ref[kind]
+        assert info["element_names"][kind] == [x[0] for x in r]
+        assert n1.tolist() == [int(x[1]) for x in r] and n2.tolist() == [int(x[2]) for x in r]
+        assert val.tolist() == [float(x[3]) for x in r]
+    # sources: nodes, waveform kind and parameters, DC value where defined
+    for kind, pos, neg, dc, wave, par in (
+            ("V", nl.vsrc_pos, nl.vsrc_neg, nl.vsrc_volts, nl.vsrc_wave, nl.vsrc_wave_par),
+            ("I", nl.isrc_pos, nl.isrc_neg, nl.isrc_amps, nl.isrc_wave, nl.isrc_wave_par)):
+        r = ref[kind]
+        assert pos.tolist() == [int(x[0]) for x in r] and neg.tolist() == [int(x[1]) for x in r]
+        for k, x in enumerate(r):
+            typ, dc_set, dc_val = x[2], int(x[3]), float(x[4])
+            vals = [float(q) for q in x[5:]]
+            if kind == "V":
+                ac, vals = vals[:2], vals[2:]
+                assert info["vsrc_ac"][k] == tuple(ac)
+            pulse, sine = vals[:7], vals[7:13]
+            assert wave[k] == WAVE[typ]
+            if typ == "PULSE":
+                assert par[k].tolist() == pulse
+            elif typ == "SIN":
+                assert par[k].tolist() == sine + [0.0]
+            if kind == "I" and (name, k) in AC_ON_CURRENT_SOURCE:
+                continue
+            if dc_set:
+                assert dc[k] == dc_val
+            else:                           # undefined there: the waveform's t = 0 value, else 0
+                want = pulse[0] if typ == "PULSE" else sine[0] if typ == "SIN" else 0.0
+                assert dc[k] == EXPLICIT_DC.get((name, kind, k), want)
+    # diodes: nodes and model names
+    r = ref["D"]
+    assert info["element_names"]["D"] == [x[0] for x in r]
+    assert nl.dio_anode.tolist() == [int(x[1]) for x in r] and nl.dio_cathode.tolist() == [int(x[2]) for x in r]
+    assert [nl.models[m].name for m in nl.dio_model] == [x[3] for x in r]
+    # MOSFETs: nodes and model names (W / L: see test_mosfet_geometry)
+    r = ref["M"]
+    assert info["element_names"]["M"] == [x[0] for x in r]
+    for col, arr in enumerate((nl.mos_d, nl.mos_g, nl.mos_s, nl.mos_b)):
+        assert arr.tolist() == [int(x[1 + col]) for x in r]
+    assert [nl.models[m].name for m in nl.mos_model] == [x[5] for x in r]
+    # model cards: the reference's cards, in its order, with KP / VTO / LAMBDA
+    cards = [m for m in nl.models if m.type is not None]
+    assert [(m.name, m.type) for m in cards] == [(x[0], x[1]) for x in ref["MODEL"]]
+    for m, x in zip(cards, ref["MODEL"]):
+        params = dict(p.split("=") for p in x[3:])
+        assert m.kp == (float(params["KP"]) if "KP" in params else None)
+        assert m.vto == (float(params["VTO"]) if "VTO" in params else None)
+        assert m.lam == (float(params["LAMBDA"]) if "LAMBDA" in params else None)
+    # a MOSFET's model entry is the card of exactly its name, or a card-less
+    # entry (tensorizeNetlist's exact-name lookup)
+    for m in nl.mos_model:
+        exact = [c for c in cards if c.name == nl.models[m].name]
+        assert (nl.models[m].type is not None) == bool(exact)
+    assert ref["counts"] == ["L", "0", "K", "0", "Q", "0", "J", "0", "E", "0", "G", "0", "H", "0", "F", "0",
+                             "T", "0", "X", "0", "S", "0"]
+
+
+def test_fills_in_what_the_reference_leaves_undefined():
+    """Diode Is / N / Vt from the diode's card (found by upper-case name, as
+    the parser finds cards), else 1e-14 A, 1, 0.02585 V; SIN is a SINE
+    waveform; AC on a current source is not a DC value."""
+    nl, _ = parse("devices")
+    assert nl.dio_is.tolist() == [1e-14, 1e-14] and nl.dio_n.tolist() == [1.05, 1.05]
+    assert nl.dio_vt.tolist() == [0.02585, 0.02585]
+    nl, info = parse("waves")
+    assert nl.vsrc_wave.tolist() == [_abi.NR_WAVE_PULSE, _abi.NR_WAVE_PULSE, _abi.NR_WAVE_SINE, 0, 0]
+    assert nl.vsrc_volts.tolist()[:3] == [0.0, 1.2, 0.5]
+    assert nl.isrc_wave.tolist() == [_abi.NR_WAVE_PULSE, _abi.NR_WAVE_SINE, 0]
+    nl, info = parse("oddities")
+    ref = read_ref("oddities")
+    assert float(ref["I"][0][4]) == 1.0 and nl.isrc_amps.tolist() == [0.0]   # reference: AC 1 -> 1 A DC
+
+
+def test_mosfet_geometry():
+    """W / L given as name-value pairs reach the device (the reference's
+    tokenizer splits "W=2u" so they never do there: every MOSFET it parses is
+    1 um x 1 um); lines without them keep the 1 um defaults."""
+    nl, _ = parse("devices")
+    assert nl.mos_w.tolist() == [2 * 1e-6, 4 * 1e-6, 1e-6, 8 * 1e-6, 1e-6]
+    assert nl.mos_l.tolist() == [1e-6, 1e-6, 1e-6, 2e-6, 1e-6]
+    assert {float(x[6]) for x in read_ref("devices")["M"]} == {1e-6}
+    nl, _ = parse("oddities")
+    assert nl.mos_w.tolist() == [3 * 1e-6, 5 * 1e-6] and nl.mos_l.tolist() == [1e-6, 1e-6]   # value x suffix
+
+
+def test_unsupported_elements_are_refused():
+    from karam_amd import spice
+    from karam_amd.nr import NrError
+    with open(os.path.join(HERE, "unsupported.sp")) as f:
+        text = f.read()
+    with pytest.raises(NrError, match="error 6: .*line 3: an inductor"):
+        spice.parse(text)
+    with pytest.raises(NrError, match="line 3: a subcircuit definition"):
+        spice.parse("t\nR1 a 0 1k\n.subckt buf a y\n.ends\n")
+    with pytest.raises(NrError, match="empty netlist"):
+        spice.parse("* only a comment\n\n")
+
+
+def test_vss_flag():
+    from karam_amd import spice
+    nl, info = spice.parse("t\nV1 vss 0 -1\nR1 a vss 1k\n")
+    assert nl.num_nodes == 1 and nl.vsrc_pos.tolist() == [0]
+    nl, info = spice.parse("t\nV1 vss 0 -1\nR1 a vss 1k\n", vss_is_net=True)
+    assert info["node_names"] == ["VSS", "A"] and nl.vsrc_pos.tolist() == [1] and nl.res_n2.tolist() == [1]
+
+
+def inverter_chain_spice(stages, vdd=1.8, cload=10e-15, pulse=(0.0, 1.8, 20e-9, 1e-9, 1e-9, 1.0, 0.0)):
+    """SPICE text of netlists.inverter_chain(stages), nodes named so that
+    first appearance gives the same ids (VDD 1, IN 2, OUT_k 2 + k)."""
+    n, p = netlists.NMOS_CARD, netlists.PMOS_CARD       # values written as exact literals
+    lines = [f"inverter chain {stages}", f".model NMOS NMOS (KP={n.kp!r} VTO={n.vto!r} LAMBDA={n.lam!r})",
+             f".model PMOS PMOS (KP={p.kp!r} VTO={p.vto!r} LAMBDA={p.lam!r})", f"VDD vdd 0 {vdd}",
+             "VIN in 0 PULSE(" + " ".join(repr(x) for x in pulse) + ")"]
+    for k in range(1, stages + 1):
+        g = "in" if k == 1 else f"o{k - 1}"
+        lines.append(f"MN{k} o{k} {g} 0 0 NMOS W=2u L=1u")
+        lines.append(f"MP{k} o{k} {g} vdd vdd PMOS W=4u L=1u")
+    for k in range(1, stages + 1):
+        lines.append(f"C{k} o{k} 0 {cload!r}")
+    return "\n".join(lines + [".end"]) + "\n"
+
+
+def test_parsed_netlist_solves_like_the_built_one(oracle_lib):
+    """The inverter chain as SPICE text parses to a netlist whose oracle DC
+    operating point and transient are bit-identical to the programmatic
+    one's."""
+    from karam_amd import spice
+    O = oracle_lib
+    a, _ = spice.parse(inverter_chain_spice(6))
+    b = netlists.inverter_chain(6)
+    assert a.num_nodes == b.num_nodes
+    for f in ("mos_d", "mos_g", "mos_s", "mos_b", "mos_w", "mos_l", "cap_n1", "cap_farads", "vsrc_pos", "vsrc_volts"):
+        np.testing.assert_array_equal(getattr(a, f), getattr(b, f))
+    Va, sa = O.solve_dc_batch(a, O.Topology(a), {}, 1)
+    Vb, sb = O.solve_dc_batch(b, O.Topology(b), {}, 1)
+    np.testing.assert_array_equal(Va, Vb)
+    assert sa["iterations"].tolist() == sb["iterations"].tolist()
+    _, Wa, _ = O.solve_transient_batch(a, O.Topology(a, transient=True), {}, 1, steps=40, probes=[3, 8])
+    _, Wb, _ = O.solve_transient_batch(b, O.Topology(b, transient=True), {}, 1, steps=40, probes=[3, 8])
+    np.testing.assert_array_equal(Wa, Wb)

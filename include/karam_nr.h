@@ -401,6 +401,40 @@ typedef struct nr_tran_stats {
 } nr_tran_stats;
 NR_API int nr_batch_get_tran_stats(const nr_batch* batch, nr_tran_stats* out, int64_t count);
 
+/* ---- AC small-signal analysis -------------------------------------------- */
+/* ACSolver::solveAC (solvers/ac_solver.h:339-383) for every instance at its
+ * operating point (the batch seed, e.g. the DC solution): Y(jw) = G + jwC
+ * with resistors, Norton V-source conductances (1e3 S), capacitors, diode
+ * conductances as buildAdmittanceMatrix takes them (:186-196: the derivative
+ * of the clamped diode current, at least 1e-15 S), MOSFET gm / gds of the
+ * instance's devices (the DC Jacobian's; the reference's own AC builder uses
+ * fixed NMOS parameters, D7), the 1e-12 diagonal, and the excitation as
+ * solveAC applies it: 1e6 S from the input node to ground with 1e6 x
+ * amplitude A injected.  Frequencies: n_points log-spaced from
+ * max(f_start, 1e-3) to max(f_stop, f_start + 1) Hz, as solveAC spaces them.
+ * Each frequency point is one complex multifrontal factorisation and solve of
+ * every instance on the GPU.  The batch's topology must hold the capacitor
+ * stamps (NR_TOPO_TRANSIENT), or the circuit have none. */
+typedef struct nr_ac_opts {
+    double f_start, f_stop;  /* Hz                                          */
+    int32_t n_points;        /* frequency points (>= 1)                     */
+    int32_t input_node;      /* 1-based excited node; 0: none               */
+    double amplitude;        /* excitation (V through the 1e6 S stamp)      */
+    int32_t n_probe;         /* 0: every node                               */
+    const int32_t* probe;    /* 1-based node ids                            */
+    int32_t record_timing;   /* as in nr_solve_opts                         */
+    int32_t pad;
+} nr_ac_opts;
+NR_API void nr_ac_opts_default(nr_ac_opts* opts);
+NR_API int nr_batch_ac(nr_batch* batch, const nr_ac_opts* opts, nr_result* result);
+/* Results of the last AC run: out[b][f][p][2] (real, imaginary node
+ * voltage), count = B * n_points * n_probe * 2; the frequencies, count =
+ * n_points; per-instance status (NR_STATUS_SINGULAR if a pivot vanished at
+ * some frequency), count = B. */
+NR_API int nr_batch_get_ac(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_ac_freqs(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_ac_status(const nr_batch* batch, int32_t* out, int64_t count);
+
 /* Per-instance results.  out arrays are caller-owned host buffers. */
 NR_API int nr_batch_get_voltages(const nr_batch* batch, double* out, int64_t count);
 enum {

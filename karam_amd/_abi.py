@@ -185,3 +185,11 @@ class NrSpiceInfo(C.Structure):
         ("tran_step", C.c_double), ("tran_stop", C.c_double), ("tran_start", C.c_double),
         ("warnings", C.c_int32), ("pad", C.c_int32),
     ]
+
+
+class NrAcOpts(C.Structure):
+    _fields_ = [
+        ("f_start", C.c_double), ("f_stop", C.c_double), ("n_points", C.c_int32), ("input_node", C.c_int32),
+        ("amplitude", C.c_double), ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)),
+        ("record_timing", C.c_int32), ("pad", C.c_int32),
+    ]

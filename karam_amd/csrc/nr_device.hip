@@ -117,8 +117,9 @@ struct DevTopo {
     // parent's pivot columns (a prefix, the maps are increasing), and the
     // inverse map parent-local row -> child row (or -1), per child
     const int *sn_pc, *inv_off, *inv;
-    // fronts whose panel is kept in HBM: offset of their panel (front-major)
-    const long long* sn_gp_off;
+    // fronts whose panel is kept in HBM: offset of their panel (front-major),
+    // for 8-byte values and for the complex path's 16-byte values
+    const long long *sn_gp_off, *sn_gp_off_c;
     long long zero_item;
     // transient: capacitors and source waveforms (NR_WAVE_*, 7 values each)
     int nC;
@@ -185,6 +186,33 @@ struct DevBatch {
 };
 
 __device__ __forceinline__ size_t IX(long long item, int b, int Bp) { return (size_t)item * (size_t)Bp + (size_t)b; }
+
+// Complex double of the AC path.  Each operation is written out with its
+// rounding steps (explicit fma, no contraction) so that the oracle, which
+// uses the same formulas, gives bit-identical results:
+//   a + b = (ar + br, ai + bi);  fma(a, b, c) = a b + c with
+//   re = fma(ar, br, fma(-ai, bi, cr)), im = fma(ar, bi, fma(ai, br, ci));
+//   a / b = ((fma(ar, br, ai bi)) / d, fma(ai, br, -(ar bi)) / d), d = fma(br, br, bi bi)
+struct cdouble {
+    double re, im;
+    __host__ __device__ constexpr cdouble(double r = 0.0, double i = 0.0) : re(r), im(i) {}
+};
+__device__ __forceinline__ cdouble operator+(cdouble a, cdouble b) { return cdouble(a.re + b.re, a.im + b.im); }
+__device__ __forceinline__ cdouble& operator+=(cdouble& a, cdouble b) { a = a + b; return a; }
+__device__ __forceinline__ cdouble operator-(cdouble a) { return cdouble(-a.re, -a.im); }
+__device__ __forceinline__ cdouble vfma(cdouble a, cdouble b, cdouble c) {
+    return cdouble(__builtin_fma(a.re, b.re, __builtin_fma(-a.im, b.im, c.re)),
+                   __builtin_fma(a.re, b.im, __builtin_fma(a.im, b.re, c.im)));
+}
+__device__ __forceinline__ cdouble operator/(cdouble a, cdouble b) {
+#pragma clang fp contract(off)
+    const double d = __builtin_fma(b.re, b.re, b.im * b.im);
+    return cdouble(__builtin_fma(a.re, b.re, a.im * b.im) / d, __builtin_fma(a.im, b.re, -(a.re * b.im)) / d);
+}
+__device__ __forceinline__ double cmag(cdouble a) {
+    const double x = __builtin_fabs(a.re), y = __builtin_fabs(a.im);
+    return x >= y ? x : y;
+}
 
 // ---------------------------------------------------------------------------
 // device helpers (reference arithmetic; FP contraction off)
@@ -1201,14 +1229,25 @@ __global__ __launch_bounds__(256) void k_assemble_hub(DevTopo T, DevBatch P, con
 }
 
 // value-type helpers of the multifrontal kernels (V = double: the FP64 path;
-// V = float: the FP32 path, same operation order)
+// V = float: the FP32 path, same operation order; V = cdouble: AC analysis
+// on complex admittances, same operation order with the complex operations
+// below)
 __device__ __forceinline__ double vfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float vfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double vabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float vabs(float x) { return __builtin_fabsf(x); }
-// the factor buffers are sized for FP64; the FP32 path uses their first half
+// a pivot the elimination treats as zero
+__device__ __forceinline__ bool vsmall(double p) { return !(vabs(p) >= 1e-25); }
+__device__ __forceinline__ bool vsmall(float p) { return !(vabs(p) >= 1e-25f); }
+__device__ __forceinline__ bool vsmall(cdouble p) { return !(cmag(p) >= 1e-25); }
+// the factor buffers are sized for FP64; the FP32 path uses their first half,
+// the complex path has buffers of twice the size
 template <typename V>
 __device__ __forceinline__ V* vp(double* p) { return reinterpret_cast<V*>(p); }
+template <typename V>
+__device__ __forceinline__ const long long* gp_offsets(const DevTopo& T) { return T.sn_gp_off; }
+template <>
+__device__ __forceinline__ const long long* gp_offsets<cdouble>(const DevTopo& T) { return T.sn_gp_off_c; }
 // Front blocks (pivot rows, update blocks and their rhs parts) of the large
 // side are stored front-major: front J's blocks for instances 0..B-1 lie
 // side by side, so the teams of one launch touch one contiguous region.
@@ -1252,7 +1291,7 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
     for (int k = 0; k < R; ++k) {
         if (k < c) {
             const V piv = F[k * R + k];
-            if (!(vabs(piv) >= V(1e-25))) sing = 1;
+            if (vsmall(piv)) sing = 1;
 #pragma unroll
             for (int i = k + 1; i < R; ++i) F[i * R + k] = F[i * R + k] / piv;
 #pragma unroll
@@ -1419,7 +1458,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const int b = act[slot];
     const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
     const int np = c * r + m * c + r;                 // panel size
-    V* Pr = GP ? vp<V>(P.gpanel) + fblk(T.sn_gp_off[J], b, np, P.B)
+    V* Pr = GP ? vp<V>(P.gpanel) + fblk(gp_offsets<V>(T)[J], b, np, P.B)
                : smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
     // front-local (i, j) with i < c or j < c; rows c.. of the pivot columns
     // have stride c in the panel
@@ -1490,7 +1529,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     int sing = 0;
     for (int k = 0; k < c; ++k) {
         const V piv = Pr[k * r + k];
-        if (!(vabs(piv) >= V(1e-25))) sing = 1;
+        if (vsmall(piv)) sing = 1;
         for (int i = k + 1 + lane; i < c; i += TEAM) Pr[i * r + k] = Pr[i * r + k] / piv;
         team_sync<TEAM>();
         for (int j = k + 1 + jl; j < c; j += SW) {
@@ -1846,6 +1885,114 @@ __global__ __launch_bounds__(1024) void k_compact(DevBatch P) {
     for (int i = lo; i < hi; ++i)
         if (!P.st[i].done) P.act[pos++] = i;
     if (tid == nt - 1) *P.nact = cnt[tid];
+}
+
+// ---------------------------------------------------------------------------
+// AC small-signal analysis (ACSolver::solveAC, solvers/ac_solver.h:339-383):
+// Y(jw) = G + jwC at the operating point, solved per frequency by the same
+// multifrontal kernels on complex values
+// ---------------------------------------------------------------------------
+// Device conductances at the operating point in vI (lane per (device,
+// instance)).  Diodes as buildAdmittanceMatrix linearises them (:186-196):
+// the derivative of diode_current_clamped as its dual numbers compute it
+// (Is exp(arg) n_vt / n_vt^2, or (Is / n_vt) e^30 past arg = 30) with the
+// netlist's unscaled thermal voltage, at least 1e-15.  MOSFETs: gm and gds of
+// the generic kernel with the instance's parameters (the DC Jacobian's).
+__global__ __launch_bounds__(256) void k_ac_eval(DevTopo T, DevBatch P, const int* __restrict__ act, int nact) {
+#pragma clang fp contract(off)
+    LANE_SETUP(4)
+    const int n = T.n;
+    auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
+    if (item < T.nD) {
+        const int i = item;
+        const double vd = V(T.d_a[i]) - V(T.d_c[i]);
+        const double Is = P.IsI[IX(i, b, Bp)], n_vt = T.d_n[i] * T.d_vtraw[i];
+        const double arg = vd / n_vt;
+        const double g = arg > 30.0 ? (Is / n_vt) * exp(30.0) : Is * (exp(arg) * (n_vt / (n_vt * n_vt)));
+        P.dg[IX(i, b, Bp)] = g < 1e-15 ? 1e-15 : g;
+    } else if (item < T.nD + T.nM) {
+        const int j = item - T.nD;
+        const int d = T.m_d[j], g = T.m_g[j], s = T.m_s[j];
+        double vgs, vds, ids, gm, gds;
+        mos_eval(V(g), V(d), V(s), T.m_w[j], T.m_l[j], P.kpI[IX(j, b, Bp)], P.vthI[IX(j, b, Bp)], T.m_lam[j],
+                 T.m_pmos[j], vgs, vds, ids, gm, gds);
+        P.mgds[IX(j, b, Bp)] = gds;
+        P.mgm[IX(j, b, Bp)] = gm;
+    }
+}
+
+// Complex A and rhs of instance b at angular frequency omega into Q's
+// factor inputs (lane per (slot, instance)).  Each slot sums its stamps in
+// emission order, real and imaginary parts alike: resistors 1/R, Norton
+// V-source conductances 1e3 and capacitors j omega C (the linear stamps),
+// then the device conductances, the 1e-12 diagonal and, at the excited
+// node, 1e6 (:364-375); the excited node's rhs is 1e6 x amplitude.
+__global__ __launch_bounds__(256) void k_ac_assemble(DevTopo T, DevBatch P, DevBatch Q, const int* __restrict__ act,
+                                                     int nact, double omega, int in_slot, int in_node,
+                                                     double g_large, double amplitude) {
+#pragma clang fp contract(off)
+    LANE_SETUP(4)
+    if (item < T.nnz_a) {
+        double re = 0.0, im = 0.0;
+        for (int k = T.lin_ptr[item]; k < T.lin_ptr[item + 1]; ++k) {
+            const int t = T.lin_term[k];
+            const int idx = term_idx(t);
+            double gr = 0.0, gi = 0.0;
+            switch (term_kind(t)) {
+            case karam::SK_RES: gr = 1.0 / P.R[(size_t)b * T.nR + idx]; break;
+            case karam::SK_CAP: gi = omega * P.cap[(size_t)b * T.nC + idx]; break;
+            default: gr = 1.0e3; break;
+            }
+            if (term_neg(t)) { re = re + (-gr); im = im + (-gi); }
+            else { re = re + gr; im = im + gi; }
+        }
+        for (int k = T.nl_ptr[item]; k < T.nl_ptr[item + 1]; ++k) {
+            const int t = T.nl_term[k];
+            const int kind = term_kind(t);
+            const double* src = kind == karam::SK_DIODE ? P.dg : kind == karam::SK_MOS_GDS ? P.mgds : P.mgm;
+            const double g = src[IX(term_idx(t), b, Bp)];
+            if (term_neg(t)) { re = re + (-g); im = im + (-0.0); }
+            else { re = re + g; im = im + 0.0; }
+        }
+        if (T.slot_diag[item]) {
+            re = re + 1e-12;
+            im = im + 0.0;
+            if (item == in_slot) {
+                re = re + g_large;
+                im = im + 0.0;
+            }
+        }
+        const int d = T.slot_dst[item];
+        const cdouble a(re, im);
+        if (d & 1) vp<cdouble>(Q.Al)[IX(d >> 1, b, Bp)] = a;
+        else vp<cdouble>(Q.pool)[IX(d >> 1, b, Bp)] = a;
+    }
+    if (item < T.n) {
+        const cdouble r = item == in_node - 1 ? cdouble(0.0 + g_large * amplitude, 0.0 + 0.0) : cdouble(0.0, 0.0);
+        const int d = T.node_dst[item];
+        if (d & 1) vp<cdouble>(Q.rhsl)[IX(d >> 1, b, Bp)] = r;
+        else vp<cdouble>(Q.pool)[IX(d >> 1, b, Bp)] = r;
+    }
+}
+
+// clear the complex pass's singular flags (thread per instance)
+__global__ void k_ac_reset(DevBatch Q, int nact) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < nact) Q.st[b].singular = 0;
+}
+
+// solution at the probed nodes of frequency point f -> out[b][f][p] (re, im);
+// singular instances are flagged in status[b]
+__global__ void k_ac_out(DevTopo T, DevBatch Q, int nact, int f, int n_freq, int n_probe,
+                         const int* __restrict__ probe, double* __restrict__ out, int* __restrict__ status) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long long)nact * n_probe) return;
+    const int b = (int)(k / n_probe), p = (int)(k % n_probe);
+    const cdouble x = vp<cdouble>(Q.xs)[IX(T.iperm[probe[p] - 1], b, Q.Bp)];
+    double* o = out + (((size_t)b * n_freq + f) * n_probe + p) * 2;
+    o[0] = x.re;
+    o[1] = x.im;
+    if (p == 0 && Q.st[b].singular) status[b] = NR_STATUS_SINGULAR;
 }
 
 __global__ __launch_bounds__(256) void k_final_kcl(DevTopo T, DevBatch P) {

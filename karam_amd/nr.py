@@ -46,6 +46,7 @@ EXPORTED = [
     "nr_seeder_param_count", "nr_seeder_get_params", "nr_seeder_set_params", "nr_seeder_loss_grad",
     "nr_spice_parse", "nr_spice_free", "nr_spice_get_netlist", "nr_spice_get_info", "nr_spice_node_name",
     "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
+    "nr_ac_opts_default", "nr_batch_ac", "nr_batch_get_ac", "nr_batch_get_ac_freqs", "nr_batch_get_ac_status",
 ]
 
 
@@ -72,6 +73,12 @@ def lib():
         L.nr_batch_autotune.argtypes = [_vp, C.c_int32, C.c_int32, C.POINTER(_abi.NrAutotuneResult)]
         L.nr_batch_get_waveform.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_waveform_times.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_ac_opts_default.argtypes = [C.POINTER(_abi.NrAcOpts)]
+        L.nr_ac_opts_default.restype = None
+        L.nr_batch_ac.argtypes = [_vp, C.POINTER(_abi.NrAcOpts), C.POINTER(_abi.NrResult)]
+        L.nr_batch_get_ac.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_ac_freqs.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_ac_status.argtypes = [_vp, C.POINTER(C.c_int32), C.c_int64]
         L.nr_spice_parse.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.POINTER(_vp)]
         L.nr_spice_free.argtypes = [_vp]
         L.nr_spice_get_netlist.argtypes = [_vp, C.POINTER(_abi.NrNetlist)]
@@ -333,6 +340,29 @@ class Batch:
                     else int(steps) + 1)
         self._wave_shape = (self.B, ring, len(self._probes))
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def ac(self, f_start=1.0, f_stop=1e9, n_points=50, input_node=0, amplitude=1.0, probes=None, record_timing=0):
+        """AC sweep of every instance at its operating point (the seed);
+        returns (freqs [F], H [B, F, n_probe] complex, status [B])."""
+        o = _abi.NrAcOpts()
+        lib().nr_ac_opts_default(C.byref(o))
+        pr = np.ascontiguousarray(probes if probes is not None else [], dtype=np.int32)
+        o.f_start, o.f_stop, o.n_points = float(f_start), float(f_stop), int(n_points)
+        o.input_node, o.amplitude = int(input_node), float(amplitude)
+        o.n_probe = len(pr)
+        o.probe = _p32(pr) if len(pr) else None
+        o.record_timing = record_timing
+        r = _abi.NrResult()
+        _check(lib().nr_batch_ac(self.h, C.byref(o), C.byref(r)), self.ctx)
+        npr = len(pr) if len(pr) else self.topo.info["n"]
+        out = np.zeros((self.B, int(n_points), npr, 2))
+        _check(lib().nr_batch_get_ac(self.h, _p64(out), out.size), self.ctx)
+        freqs = np.zeros(int(n_points))
+        _check(lib().nr_batch_get_ac_freqs(self.h, _p64(freqs), freqs.size), self.ctx)
+        st = np.zeros(self.B, dtype=np.int32)
+        _check(lib().nr_batch_get_ac_status(self.h, st.ctypes.data_as(C.POINTER(C.c_int32)), self.B), self.ctx)
+        self._ac_result = {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+        return freqs, out[..., 0] + 1j * out[..., 1], st
 
     def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
         """Choose the large-front launch shape per front-order bucket by timing

@@ -299,6 +299,17 @@ static void iv_push(ivec* v, int32_t x) {
     v->a[v->n++] = x;
 }
 
+/* AC conductance of a diode (buildAdmittanceMatrix, solvers/ac_solver.h:
+ * 186-196): the derivative of diode_current_clamped as its dual numbers
+ * compute it (the dual quotient vd / n_vt carries n_vt / n_vt^2; past
+ * arg = 30 the continuation's slope (Is / n_vt) e^30), at least 1e-15. */
+static double ac_diode_g(double vd, double Is, double N, double Vt) {
+    const double n_vt = N * Vt;
+    const double arg = vd / n_vt;
+    const double g = arg > 30.0 ? (Is / n_vt) * exp(30.0) : Is * (exp(arg) * (n_vt / (n_vt * n_vt)));
+    return g < 1e-15 ? 1e-15 : g;
+}
+
 /* Emit the DC triplet coordinates in stampSoABlock order
  * (physics/circuitsim.h:2413-2751, h = 0) followed by the gmin diagonal
  * (addDiagonalConditioning, :2857-2860).  When vals != NULL the values and
@@ -308,6 +319,8 @@ typedef struct {
     double* vals; int32_t nv;      /* optional values */
     double* rhs;
     int f32;                       /* FP32 path: device values from the FP32 evaluation */
+    int ac;                        /* AC admittance: 1 its real part, 2 its imaginary part */
+    double omega;                  /* AC angular frequency */
     int32_t n_lin;                 /* stamps of the linear elements (R, V, C), emitted first */
 } emitter;
 static inline void emit(emitter* e, int32_t r, int32_t c, double val) {
@@ -337,7 +350,7 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     /* (1) resistors :2416-2438 */
     for (int32_t i = 0; i < nl->n_res; ++i) {
         int32_t n1 = nl->res_n1[i], n2 = nl->res_n2[i];
-        double g = 1.0 / nl->res_ohms[i];
+        double g = e->ac == 2 ? 0.0 : 1.0 / nl->res_ohms[i];
         if (n1 != 0 && n2 != 0) {
             emit(e, n1 - 1, n1 - 1, g); emit(e, n2 - 1, n2 - 1, g);
             emit(e, n1 - 1, n2 - 1, -g); emit(e, n2 - 1, n1 - 1, -g);
@@ -348,7 +361,7 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     for (int32_t i = 0; i < nl->n_vsrc; ++i) {
         int32_t p = nl->vsrc_pos[i], q = nl->vsrc_neg[i];
         double V = (tc && tc->vsrc) ? tc->vsrc[i] : nl->vsrc_volts[i];
-        double g = 1.0e3, ieq = V * g;
+        double g = e->ac == 2 ? 0.0 : 1.0e3, ieq = V * g;
         if (p != 0 && q != 0) {
             emit(e, p - 1, p - 1, g); emit(e, q - 1, q - 1, g);
             emit(e, p - 1, q - 1, -g); emit(e, q - 1, p - 1, -g);
@@ -379,7 +392,9 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
         int32_t a = nl->cap_n1[i], b = nl->cap_n2[i];
         const double C = nl->cap_farads[i];
         double G = 0.0, ra = 0.0;   /* ra: added to rhs[a], subtracted from rhs[b] */
-        if (!tc->coords_only) {
+        if (e->ac) {
+            G = e->ac == 2 ? e->omega * C : 0.0;         /* j omega C (ac_solver.h:160-167) */
+        } else if (!tc->coords_only) {
             if (tc->method == ORC_INT_TRAP) {
                 G = (2.0 * C) / tc->h;
                 ra = -(-G * tc->hv0[i] - tc->hi0[i]);
@@ -406,7 +421,10 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     for (int32_t i = 0; i < s->nD; ++i) {
         int32_t a = s->da[i], c = s->dc[i];
         double g = 0.0, ieq = 0.0;
-        if (e->vals && e->f32) {
+        if (e->vals && e->ac) {
+            g = e->ac == 2 ? 0.0 : ac_diode_g(volt(v, n, a) - volt(v, n, c), s->dIs[i], s->dN[i],
+                                               nl->dio_vt[s->d_src[i]]);
+        } else if (e->vals && e->f32) {
             g = diode_g_f32(volt(v, n, a) - volt(v, n, c), s->dIs[i], s->dN[i], s->dVt[i]);
         } else if (e->vals) {
             double vd, id, gd;
@@ -427,7 +445,9 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
     for (int32_t i = 0; i < s->nM; ++i) {
         int32_t d = s->md[i], g = s->mg[i], so = s->ms[i], b = s->mb[i];
         double gm = 0, gmb = 0, gds = 0, ieq = 0;
-        if (e->vals && e->f32) {
+        if (e->vals && e->ac == 2) {
+            /* no imaginary part */
+        } else if (e->vals && e->f32) {
             const double vs_ = volt(v, n, so);
             float gm32, gds32;
             mos_g_f32(volt(v, n, g) - vs_, volt(v, n, d) - vs_, s->mW[i], s->mL[i], s->mKp[i], s->mVth[i],
@@ -467,7 +487,7 @@ static void stamp_all(const nr_netlist* nl, const orc_soa* s, const double* v,
         }
     }
     /* gmin diagonal :2857-2860 */
-    for (int32_t i = 0; i < n; ++i) emit(e, i, i, e->f32 ? (double)(float)gmin : gmin);
+    for (int32_t i = 0; i < n; ++i) emit(e, i, i, e->ac == 2 ? 0.0 : e->f32 ? (double)(float)gmin : gmin);
 }
 
 /* Sparse adjacency with O(1) membership (bitset) for graph elimination. */
@@ -1906,3 +1926,211 @@ int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, doubl
 
 /* diode_model_pair current used by the KCL audit */
 double orc_diode_i_clamped(double vd, double Is, double N, double Vt) { return diode_i_clamped(vd, Is, N, Vt); }
+
+/* ---- AC small-signal analysis (ACSolver::solveAC, solvers/ac_solver.h:339-383) ----
+ * Complex arithmetic with explicit rounding steps, the formulas the GPU's
+ * complex path uses:
+ *   a + b = (ar + br, ai + bi)
+ *   fma(a, b, c) = (fma(ar, br, fma(-ai, bi, cr)), fma(ar, bi, fma(ai, br, ci)))
+ *   a / b = (fma(ar, br, ai bi) / d, fma(ai, br, -(ar bi)) / d), d = fma(br, br, bi bi)
+ *   a pivot is zero when max(|re|, |im|) < 1e-25 */
+typedef struct { double re, im; } cplx;
+static inline cplx c_add(cplx a, cplx b) { cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cplx c_neg(cplx a) { cplx r = {-a.re, -a.im}; return r; }
+static inline cplx c_fma(cplx a, cplx b, cplx c) {
+    cplx r = {fma(a.re, b.re, fma(-a.im, b.im, c.re)), fma(a.re, b.im, fma(a.im, b.re, c.im))};
+    return r;
+}
+static inline cplx c_div(cplx a, cplx b) {
+    const double d = fma(b.re, b.re, b.im * b.im);
+    cplx r = {fma(a.re, b.re, a.im * b.im) / d, fma(a.im, b.re, -(a.re * b.im)) / d};
+    return r;
+}
+static inline double c_mag(cplx a) {
+    const double x = fabs(a.re), y = fabs(a.im);
+    return x >= y ? x : y;
+}
+
+/* orc_lu_multifrontal on complex values (same operation order) */
+static int lu_multifrontal_c(const orc_topo* t, const cplx* a_vals, const cplx* rhs, cplx* x) {
+    const int32_t n = t->n, ns = t->ns;
+    int64_t* u_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* cb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t* rcb_off = (int64_t*)xmalloc(8 * (size_t)ns);
+    int64_t ut = 0, ct = 0, rt = 0;
+    int32_t rmax = 1;
+    for (int32_t J = 0; J < ns; ++J) {
+        int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m;
+        u_off[J] = ut; ut += (int64_t)c * r;
+        cb_off[J] = ct; ct += (int64_t)m * m;
+        rcb_off[J] = rt; rt += m;
+        if (r > rmax) rmax = r;
+    }
+    const size_t cz = sizeof(cplx);
+    cplx* U = (cplx*)xmalloc(cz * (size_t)(ut + 1));
+    cplx* CB = (cplx*)xmalloc(cz * (size_t)(ct + 1));
+    cplx* RCB = (cplx*)xmalloc(cz * (size_t)(rt + 1));
+    cplx* y = (cplx*)xmalloc(cz * (size_t)n);
+    cplx* xp = (cplx*)xmalloc(cz * (size_t)n);
+    cplx* F = (cplx*)xmalloc(cz * (size_t)rmax * (size_t)rmax);
+    cplx* fr = (cplx*)xmalloc(cz * (size_t)rmax);
+    const cplx zero = {0.0, 0.0};
+    int rc = NR_OK;
+    for (int32_t J = 0; J < ns; ++J) {
+        const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+        for (int32_t e = 0; e < r * r; ++e) F[e] = zero;
+        for (int32_t e = 0; e < r; ++e) fr[e] = zero;
+        for (int32_t q = t->sn_a_ptr[J]; q < t->sn_a_ptr[J + 1]; ++q) {
+            cplx* d = &F[t->sn_a_li[q] * r + t->sn_a_lj[q]];
+            *d = c_add(*d, a_vals[t->sn_a_slot[q]]);
+        }
+        for (int32_t k = 0; k < c; ++k) fr[k] = c_add(fr[k], rhs[t->perm[f + k]]);
+        for (int32_t q = t->sn_child_ptr[J]; q < t->sn_child_ptr[J + 1]; ++q) {
+            const int32_t C = t->sn_child[q], mc = t->sn_nrows[C];
+            const int32_t* Rc = t->sn_rows + t->sn_row_off[C];
+            int32_t* rel = (int32_t*)xmalloc(4 * (size_t)(mc + 1));
+            for (int32_t a = 0; a < mc; ++a) rel[a] = front_local(t, J, Rc[a]);
+            const cplx* cb = CB + cb_off[C];
+            for (int32_t a = 0; a < mc; ++a)
+                for (int32_t b = 0; b < mc; ++b) F[rel[a] * r + rel[b]] = c_add(F[rel[a] * r + rel[b]], cb[a * mc + b]);
+            for (int32_t a = 0; a < mc; ++a) fr[rel[a]] = c_add(fr[rel[a]], RCB[rcb_off[C] + a]);
+            free(rel);
+        }
+        for (int32_t k = 0; k < c; ++k) {
+            const cplx piv = F[k * r + k];
+            if (!(c_mag(piv) >= 1e-25)) rc = NR_STATUS_SINGULAR;
+            for (int32_t i = k + 1; i < r; ++i) F[i * r + k] = c_div(F[i * r + k], piv);
+            for (int32_t i = k + 1; i < r; ++i) {
+                const cplx l = c_neg(F[i * r + k]);
+                for (int32_t j = k + 1; j < r; ++j) F[i * r + j] = c_fma(l, F[k * r + j], F[i * r + j]);
+                fr[i] = c_fma(l, fr[k], fr[i]);
+            }
+        }
+        memcpy(U + u_off[J], F, cz * (size_t)c * (size_t)r);
+        for (int32_t k = 0; k < c; ++k) y[f + k] = fr[k];
+        for (int32_t a = 0; a < m; ++a) {
+            for (int32_t b = 0; b < m; ++b) CB[cb_off[J] + a * m + b] = F[(c + a) * r + c + b];
+            RCB[rcb_off[J] + a] = fr[c + a];
+        }
+    }
+    if (rc == NR_OK) {
+        for (int32_t J = ns - 1; J >= 0; --J) {
+            const int32_t c = t->sn_ncols[J], m = t->sn_nrows[J], r = c + m, f = t->sn_first[J];
+            const cplx* Uj = U + u_off[J];
+            const int32_t* R = t->sn_rows + t->sn_row_off[J];
+            for (int32_t k = 0; k < c; ++k) {
+                cplx s = y[f + k];
+                for (int32_t a = 0; a < m; ++a) s = c_fma(c_neg(Uj[k * r + c + a]), xp[R[a]], s);
+                fr[k] = s;
+            }
+            for (int32_t k = c - 1; k >= 0; --k) {
+                const cplx xk = c_div(fr[k], Uj[k * r + k]);
+                xp[f + k] = xk;
+                for (int32_t i = 0; i < k; ++i) fr[i] = c_fma(c_neg(Uj[i * r + k]), xk, fr[i]);
+            }
+        }
+        for (int32_t i = 0; i < n; ++i) x[t->perm[i]] = xp[i];
+    } else {
+        for (int32_t i = 0; i < n; ++i) x[i] = zero;
+    }
+    free(u_off); free(cb_off); free(rcb_off); free(U); free(CB); free(RCB); free(y); free(xp); free(F); free(fr);
+    return rc;
+}
+
+/* log-spaced frequencies as solveAC spaces them (:353-359) */
+int orc_ac_freqs(const orc_ac_opts* o, double* freqs) {
+    if (o->n_points <= 0) return NR_ERR_INVALID;
+    const double logStart = log10(dmax_ref(o->f_start, 1e-3));
+    const double logStop = log10(dmax_ref(o->f_stop, o->f_start + 1.0));
+    const int32_t den = o->n_points - 1 > 1 ? o->n_points - 1 : 1;
+    for (int32_t i = 0; i < o->n_points; ++i) {
+        const double logF = logStart + (logStop - logStart) * i / den;
+        freqs[i] = pow(10.0, logF);
+    }
+    return NR_OK;
+}
+
+/* AC sweep of one instance at its operating point v_op: Y(jw) = G + jwC
+ * stamped in the transient pattern's emission order (real and imaginary parts
+ * alike), the 1e-12 diagonal, then the 1e6 S excitation at the input node
+ * with 1e6 x amplitude injected (:364-375); out[f][p][2]. */
+int orc_solve_ac(const nr_netlist* nl, const orc_topo* t, const double* v_op, const orc_ac_opts* o,
+                 int32_t n_probe, const int32_t* probe, double* out, int32_t* status) {
+    const int32_t n = nl->num_nodes;
+    if (!t || !t->transient || o->n_points <= 0 || o->input_node < 0 || o->input_node > n) return NR_ERR_INVALID;
+    orc_soa* s = orc_tensorize(nl);
+    const int32_t F = o->n_points;
+    double* freqs = (double*)xmalloc(sizeof(double) * (size_t)F);
+    orc_ac_freqs(o, freqs);
+    double* vre = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* vim = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* are = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    double* aim = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    cplx* A = (cplx*)xmalloc(sizeof(cplx) * (size_t)t->nnz_a);
+    cplx* rhs = (cplx*)xmalloc(sizeof(cplx) * (size_t)n);
+    cplx* x = (cplx*)xmalloc(sizeof(cplx) * (size_t)n);
+    int32_t in_slot = -1;
+    if (o->input_node > 0)
+        for (int32_t k = t->row_ptr[o->input_node - 1]; k < t->row_ptr[o->input_node]; ++k)
+            if (t->col_idx[k] == o->input_node - 1) in_slot = k;
+    tran_ctx tc = {1.0, ORC_INT_GEAR1, NULL, NULL, NULL, NULL, NULL, 0};
+    emitter e;
+    memset(&e, 0, sizeof(e));
+    e.vals = vre;
+    e.ac = 1;
+    stamp_all(nl, s, v_op, 1e-12, &e, &tc);
+    int rc = e.nv == t->n_stamps ? NR_OK : NR_ERR_TOPOLOGY;
+    orc_assemble(t, vre, are);
+    *status = NR_STATUS_CONVERGED;
+    for (int32_t f = 0; f < F && rc == NR_OK; ++f) {
+        const double omega = 2.0 * 3.14159265358979323846 * freqs[f];
+        memset(&e, 0, sizeof(e));
+        e.vals = vim;
+        e.ac = 2;
+        e.omega = omega;
+        stamp_all(nl, s, v_op, 1e-12, &e, &tc);
+        orc_assemble(t, vim, aim);
+        for (int32_t k = 0; k < t->nnz_a; ++k) { A[k].re = are[k]; A[k].im = aim[k]; }
+        if (in_slot >= 0) { A[in_slot].re = A[in_slot].re + 1e6; A[in_slot].im = A[in_slot].im + 0.0; }
+        for (int32_t i = 0; i < n; ++i) { rhs[i].re = 0.0; rhs[i].im = 0.0; }
+        if (o->input_node > 0) {
+            rhs[o->input_node - 1].re = 0.0 + 1e6 * o->amplitude;
+            rhs[o->input_node - 1].im = 0.0 + 0.0;
+        }
+        if (lu_multifrontal_c(t, A, rhs, x) != NR_OK) *status = NR_STATUS_SINGULAR;
+        for (int32_t p = 0; p < n_probe; ++p) {
+            out[((size_t)f * n_probe + p) * 2] = x[probe[p] - 1].re;
+            out[((size_t)f * n_probe + p) * 2 + 1] = x[probe[p] - 1].im;
+        }
+    }
+    free(freqs); free(vre); free(vim); free(are); free(aim); free(A); free(rhs); free(x);
+    orc_soa_free(s);
+    return rc;
+}
+
+int orc_solve_ac_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                       const double* v_op, const orc_ac_opts* o, int32_t threads, int32_t n_probe,
+                       const int32_t* probe, double* out, int32_t* status) {
+    if (!t || B <= 0) return NR_ERR_INVALID;
+    const int32_t n = nl->num_nodes;
+    int bad = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+    for (int32_t b = 0; b < B; ++b) {
+        nr_netlist inst = *nl;
+        if (fields) {
+            if (fields[NR_FIELD_RES_OHMS]) inst.res_ohms = fields[NR_FIELD_RES_OHMS] + (size_t)b * nl->n_res;
+            if (fields[NR_FIELD_DIO_IS]) inst.dio_is = fields[NR_FIELD_DIO_IS] + (size_t)b * nl->n_diode;
+            if (fields[NR_FIELD_MOS_KP]) inst.mos_kp = fields[NR_FIELD_MOS_KP] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_MOS_VTO]) inst.mos_vto = fields[NR_FIELD_MOS_VTO] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
+            if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
+            if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+        }
+        bad |= orc_solve_ac(&inst, t, v_op + (size_t)b * n, o, n_probe, probe,
+                            out + (size_t)b * o->n_points * n_probe * 2, &status[b]) != NR_OK;
+    }
+    return bad ? NR_ERR_INVALID : NR_OK;
+}

@@ -173,6 +173,21 @@ int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B
                               const orc_tran_opts* opts, int32_t threads, double* v_io, int32_t n_probe,
                               const int32_t* probe, double* wave_t, double* wave, orc_tran_out* outs);
 
+/* --- AC small-signal analysis (ACSolver::solveAC) ------------------------------ */
+typedef struct orc_ac_opts {
+    double f_start, f_stop;
+    int32_t n_points;
+    int32_t input_node;   /* 1-based; 0 none */
+    double amplitude;
+} orc_ac_opts;
+int orc_ac_freqs(const orc_ac_opts* o, double* freqs);
+/* out[f][p][2] (re, im); status NR_STATUS_SINGULAR when a pivot vanished */
+int orc_solve_ac(const nr_netlist* nl, const orc_topo* t, const double* v_op, const orc_ac_opts* o,
+                 int32_t n_probe, const int32_t* probe, double* out, int32_t* status);
+int orc_solve_ac_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                       const double* v_op, const orc_ac_opts* o, int32_t threads, int32_t n_probe,
+                       const int32_t* probe, double* out, int32_t* status);
+
 int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
                          double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,
                          double* m_w, double* m_l, double* m_kp, double* m_vth, double* m_lam,

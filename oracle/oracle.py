@@ -38,6 +38,11 @@ class OrcTranOut(C.Structure):
                 ("kcl", C.c_double), ("t_end", C.c_double), ("h_last", C.c_double)]
 
 
+class OrcAcOpts(C.Structure):
+    _fields_ = [("f_start", C.c_double), ("f_stop", C.c_double), ("n_points", C.c_int32),
+                ("input_node", C.c_int32), ("amplitude", C.c_double)]
+
+
 class OrcDcOut(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
                 ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
@@ -69,6 +74,9 @@ def lib():
         L.orc_sine_value.argtypes = [_f64p, C.c_double]
         L.orc_sine_value.restype = C.c_double
         L.orc_wave_capacity.argtypes = [C.POINTER(OrcTranOpts)]
+        L.orc_ac_freqs.argtypes = [C.POINTER(OrcAcOpts), _f64p]
+        L.orc_solve_ac_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32, C.POINTER(_f64p), _f64p,
+                                         C.POINTER(OrcAcOpts), C.c_int32, C.c_int32, _i32p, _f64p, _i32p]
         L.orc_arbiter_analyze.argtypes = [C.POINTER(_abi.NrNetlist), _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p,
                                           _f64p, _f64p, C.POINTER(C.c_int32), _f64p]
         L.orc_wave_capacity.restype = C.c_int32
@@ -246,6 +254,34 @@ def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, step
     if full:
         return v, wave, stats, times
     return v, wave, stats
+
+
+def solve_ac_batch(nl, topo: Topology, fields: dict, B: int, v_op, f_start=1.0, f_stop=1e9, n_points=50,
+                   input_node=0, amplitude=1.0, probes=None, threads=0):
+    """AC sweep of B instances at their operating points v_op [B, n];
+    returns (freqs [F], H [B, F, n_probe] complex, status [B])."""
+    c, keep = nl.to_c()
+    arrs = (_f64p * _abi.N_FIELDS)()
+    hold = []
+    for fid in range(_abi.N_FIELDS):
+        if fid in fields:
+            a = np.ascontiguousarray(fields[fid][:B], dtype=np.float64)
+            hold.append(a)
+            arrs[fid] = _p64(a)
+        else:
+            arrs[fid] = None
+    v = np.ascontiguousarray(v_op, dtype=np.float64).reshape(B, nl.num_nodes)
+    pr = np.ascontiguousarray(probes if probes is not None else np.arange(1, nl.num_nodes + 1), np.int32)
+    o = OrcAcOpts(f_start, f_stop, n_points, input_node, amplitude)
+    freqs = np.zeros(n_points)
+    lib().orc_ac_freqs(C.byref(o), _p64(freqs))
+    out = np.zeros((B, n_points, len(pr), 2))
+    status = np.zeros(B, dtype=np.int32)
+    rc = lib().orc_solve_ac_batch(C.byref(c), topo.h, B, arrs, _p64(v), C.byref(o), threads, len(pr), _p32(pr),
+                                  _p64(out), _p32(status))
+    if rc != 0:
+        raise ValueError(f"orc_solve_ac_batch: invalid arguments (status {rc})")
+    return freqs, out[..., 0] + 1j * out[..., 1], status
 
 
 def eval_diodes(a, c, Is, N, Vt, v):

@@ -48,6 +48,8 @@ def lib():
         L.ref_diode_i_clamped.argtypes = [C.c_double] * 4
         L.ref_diode_i_clamped.restype = C.c_double
         L.ref_spice_dump.argtypes = [C.c_char_p, C.c_char_p, C.c_int64]
+        L.ref_ac.argtypes = [C.POINTER(_abi.NrNetlist), _f64p, C.c_double, C.c_double, C.c_int32, C.c_int32,
+                             C.c_double, _f64p, _f64p]
         L.ref_spice_dump.restype = C.c_int64
         _LIB = L
     return _LIB
@@ -170,3 +172,15 @@ def spice_dump(text: str) -> str:
     buf = C.create_string_buffer(int(need) + 1)
     lib().ref_spice_dump(raw, buf, need + 1)
     return buf.value.decode()
+
+
+def solve_ac(nl, v_op, f_start, f_stop, n_points, input_node=0, amplitude=1.0):
+    """ACSolver::solveAC on the reference's own code: (freqs [F], H [F, n])."""
+    c, keep = nl.to_c()
+    v = np.ascontiguousarray(v_op, dtype=np.float64)
+    freqs = np.zeros(n_points)
+    out = np.zeros((n_points, nl.num_nodes, 2))
+    rc = lib().ref_ac(C.byref(c), _p64(v), f_start, f_stop, n_points, input_node, amplitude, _p64(freqs), _p64(out))
+    if rc:
+        raise RuntimeError("ref_ac failed")
+    return freqs, out[..., 0] + 1j * out[..., 1]

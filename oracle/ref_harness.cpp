@@ -1,6 +1,6 @@
 // ref_harness.cpp — thin C entry points over the reference engine's own
 // compilable translation units (math/sparse_lu.cpp, math/linalg.h,
-// netlist/spice_parser.cpp,
+// netlist/spice_parser.cpp, solvers/ac_solver.h,
 // infrastructure/factor_graph.cpp, infrastructure/treewidth_analyzer.cpp,
 // tensors/physics_tensors.cpp, tensors/kernel_dispatch.cpp,
 // physics/device_physics.h), compiled from /root/reference by oracle/Makefile
@@ -21,6 +21,7 @@
 #include "math/sparse_lu.h"
 #include "netlist/circuit.h"
 #include "netlist/spice_parser.h"
+#include "solvers/ac_solver.h"
 #include "physics/device_physics.h"
 #include "tensors/kernel_dispatch.h"
 #include "tensors/physics_tensors.h"
@@ -249,6 +250,24 @@ int64_t ref_spice_dump(const char* text, char* out, int64_t cap) {
         (int)g.transmissionLines.size(), (int)r.netlist.instances.size(), r.subcircuitsCreated);
     if ((int64_t)o.size() < cap) std::memcpy(out, o.c_str(), o.size() + 1);
     return (int64_t)o.size();
+}
+
+// ACSolver::solveAC (solvers/ac_solver.h:339-383, dense complex LU with
+// partial pivoting) at the operating point v_op; out[f][node][2].
+int ref_ac(const nr_netlist* nl, const double* v_op, double f_start, double f_stop, int32_t n_points,
+           int32_t input_node, double amplitude, double* freqs, double* out) {
+    TensorNetlist t = make_netlist(nl);
+    std::vector<double> dc(v_op, v_op + nl->num_nodes);
+    ACSolver::ACResult r = ACSolver::solveAC(t, dc, f_start, f_stop, n_points, input_node, amplitude);
+    if ((int)r.frequencies.size() != n_points) return 1;
+    for (int f = 0; f < n_points; ++f) {
+        freqs[f] = r.frequencies[f];
+        for (int i = 0; i < nl->num_nodes; ++i) {
+            out[((size_t)f * nl->num_nodes + i) * 2] = r.nodeVoltages[f][i].real();
+            out[((size_t)f * nl->num_nodes + i) * 2 + 1] = r.nodeVoltages[f][i].imag();
+        }
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -338,6 +338,69 @@ def logic_workload(a):
     return 0
 
 
+def ac_workload(a):
+    """AC small-signal sweeps of the benchmark mesh (64 x 64 R-diode mesh with
+    a 2 pF capacitor at every node): DC operating points of the Monte-Carlo
+    instances, then a.ac_points log-spaced frequencies from 1 Hz to 10 GHz,
+    excited at node 1 as solveAC excites it.  One frequency point of one
+    instance is one complex sparse factorisation and solve."""
+    from karam_amd import _abi, netlists, nr
+    from karam_amd.netlists import _f64, _i32
+    B = a.instances if a.instances != 4096 else 1024
+    nl = netlists.rd_mesh(a.side, seed=1)
+    n = nl.num_nodes
+    nl.cap_n1, nl.cap_n2, nl.cap_farads = _i32(np.arange(1, n + 1)), _i32(np.zeros(n)), _f64(np.full(n, 2e-12))
+    f = netlists.mc_fields(nl, B, seed=1000)
+    ctx = nr.Context(0)
+    T = nr.Topology(nl, ctx)
+    D = nr.Batch(ctx, T, B)
+    D.set_fields(f)
+    D.solve_dc()
+    V = D.voltages()
+    D.close()
+    Tt = nr.Topology(nl, ctx, transient=True)
+    bt = nr.Batch(ctx, Tt, B)
+    bt.set_fields(f)
+    bt.set_seed(V)
+    probes = [1, 2, n // 2, n]
+    kw = dict(f_start=1.0, f_stop=10e9, input_node=1, amplitude=1.0, probes=probes)
+    for _ in range(a.warmup):
+        bt.ac(n_points=2, **kw)
+    t0 = time.perf_counter()
+    freqs, H, st = bt.ac(n_points=a.ac_points, record_timing=2, **kw)
+    wall = time.perf_counter() - t0
+    kst = bt.kernel_stats()
+    points = B * a.ac_points
+    log(f"ac: {points} instance-frequency points in {wall:.2f} s, singular {(st != 0).sum()}")
+    out = {
+        "metric": "AC instance-frequency points/sec", "value": points / wall, "unit": "points/s", "n_gpus": 1,
+        "steps": a.ac_points, "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.ac_points, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex f64",
+        "data": "synthetic: seeded R-diode mesh with 2 pF per node, per-instance R and IS sigma 5 %",
+        "config": {"workload": f"rdmesh{a.side} + C, AC sweep 1 Hz - 10 GHz, {a.ac_points} points",
+                   "instances": B, "nodes": n, "nnz_f": Tt.info["nnz_f"], "max_front": Tt.info["max_front"]},
+        "kernels": kernel_table(kst), "roofline": None, "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        m = min(B, threads)
+        To = O.Topology(nl, transient=True)
+        t0 = time.perf_counter()
+        fo, Ho, so = O.solve_ac_batch(nl, To, {k: v[:m] for k, v in f.items()}, m, V[:m], n_points=a.ac_points,
+                                      threads=threads, **kw)
+        t_cpu = time.perf_counter() - t0
+        err = np.abs(H[:m] - Ho) / (1e-9 * np.abs(Ho) + 1e-12 * np.abs(Ho).max())
+        out["cpu_baseline"] = {"value": m * a.ac_points / t_cpu, "unit": "points/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} of {B} instances, all {a.ac_points} points, oracle (complex "
+                                         f"multifrontal LU on the reference fill pattern), one instance per thread"}
+        out["parity_sample"] = {"instances": m, "max_err_over_tol": float(err.max()),
+                                "tolerance": "1e-9 rel + 1e-12 x max |H|"}
+        log(f"ac cpu baseline {m * a.ac_points / t_cpu:.1f} points/s; parity {out['parity_sample']}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def mixed_workload(a):
     """Uneven sizes: a.mixed_instances instances whose node counts are
     log-uniform in [200, 50000], the family of instance i being i mod 3 (the
@@ -437,7 +500,8 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
-    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed"], default="rdmesh64")
+    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac"], default="rdmesh64")
+    ap.add_argument("--ac-points", type=int, default=50)
     ap.add_argument("--mixed-instances", type=int, default=10000)
     ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--logic-nodes", type=int, default=2000)
@@ -457,6 +521,8 @@ def main():
         return logic_workload(a)
     if a.workload == "mixed":
         return mixed_workload(a)
+    if a.workload == "ac":
+        return ac_workload(a)
 
     from karam_amd import _abi, netlists, nr
 

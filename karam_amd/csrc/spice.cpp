@@ -337,15 +337,18 @@ private:
         return NR_OK;
     }
 
-    // model entries: the cards (in the reference's upper-case key order, under
-    // the name their last .model line gave), then each referenced name with no
+    // model entries: the cards (one per upper-case name, under the name its
+    // last .model line gave, in the order of the netlist's card map: by exact
+    // name, netlist/circuit.h:1078-1080), then each referenced name with no
     // card of exactly that name (tensorizeNetlist looks cards up by exact
     // name, tensors/physics_tensors.cpp:424-441)
     void finish() {
         o_.num_nodes = (int32_t)o_.node_names.size();
+        std::map<std::string, const Card*> final_cards;
+        for (const auto& kv : cards_) final_cards[kv.second.name] = &kv.second;
         std::map<std::string, int32_t> by_name;
-        for (const auto& kv : cards_) {
-            const Card& c = kv.second;
+        for (const auto& kv : final_cards) {
+            const Card& c = *kv.second;
             by_name[c.name] = (int32_t)o_.model_name.size();
             o_.model_name.push_back(c.name);
             o_.model_type.push_back(c.type);

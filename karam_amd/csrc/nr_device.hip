@@ -1180,51 +1180,89 @@ rhs:
 // The A entries and FP64 rhs rows with more than HUB_TERMS device terms
 // (a supply rail), one thread per (item, instance): the same sums in the
 // same order as k_assemble / k_assemble32, with the loads batched.
-__global__ __launch_bounds__(256) void k_assemble_hub(DevTopo T, DevBatch P, const int* __restrict__ act,
-                                                      int nact_grid, const double* __restrict__ gmin_tab,
-                                                      int gmin_last) {
+// Slots and rhs rows with many device terms (a supply rail's diagonal holds
+// one per device on the rail), one block of 16 wavefronts per (item, 64
+// instances): the wavefronts load a chunk of HUB_SUM_CHUNK terms at once into
+// LDS, then the first adds the chunk in list order, lane by lane (each lane an
+// instance), so every value is the sequential sum k_assemble would form.
+constexpr int HUB_SUM_CHUNK = 256;
+constexpr int HUB_SUM_THREADS = 1024;
+
+template <typename A>
+__device__ __forceinline__ A hub_sum(A acc, const int* __restrict__ terms, int k0, int k1, int lane, int w,
+                                     bool on, const double* const* src, size_t b_off, int Bp, double* lds) {
+    A(*buf)[64] = reinterpret_cast<A(*)[64]>(lds);
+    constexpr int PER = HUB_SUM_CHUNK / (HUB_SUM_THREADS / 64);
+    for (int base = k0; base < k1; base += HUB_SUM_CHUNK) {
+        const int cnt = min(HUB_SUM_CHUNK, k1 - base);
+        A x[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = w * PER + u;
+            const int t = terms[base + min(q, cnt - 1)];
+            const A g = reinterpret_cast<const A*>(src[term_kind(t) & 3])[IX(term_idx(t), 0, Bp) + b_off];
+            x[u] = term_neg(t) ? -g : g;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) buf[w * PER + u][lane] = x[u];
+        __syncthreads();
+        if (w == 0 && on)
+            for (int q = 0; q < cnt; ++q) acc = acc + buf[q][lane];
+        __syncthreads();
+    }
+    return acc;
+}
+
+__global__ __launch_bounds__(HUB_SUM_THREADS) void k_assemble_hub(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                                 int nact_grid, const double* __restrict__ gmin_tab,
+                                                                 int gmin_last) {
 #pragma clang fp contract(off)
     const int nact = nact_grid > 0 ? nact_grid : *P.nact;
-    LANE_SETUP(4)
-    const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
+    const int item = blockIdx.x;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int slot_ = blockIdx.y * 64 + lane;
+    const bool on = slot_ < nact;
+    const int b = on ? act[slot_] : act[0];
+    const int Bp = P.Bp;
+    __shared__ double lds[HUB_SUM_CHUNK * 64];
+    // term sources by kind: the slot terms' SK_DIODE / SK_MOS_GDS / SK_MOS_GM
+    // values and the rhs terms' TK_DIODE / TK_MOS values (kind & 3 indexes)
+    const double* slot_src[4];
+    const double* row_src[4];
+    slot_src[karam::SK_DIODE & 3] = P.dg;
+    slot_src[karam::SK_MOS_GDS & 3] = P.mgds;
+    slot_src[karam::SK_MOS_GM & 3] = P.mgm;
+    row_src[karam::TK_DIODE & 3] = P.dieq;
+    row_src[karam::TK_MOS & 3] = P.mieq;
     if (item < T.n_hub_slot) {
         const int sl = T.hub_slot[item];
         const int d = T.slot_dst[sl];
         if (P.p32) {
-            const float* dg = reinterpret_cast<const float*>(P.dg);
-            const float* mgds = reinterpret_cast<const float*>(P.mgds);
-            const float* mgm = reinterpret_cast<const float*>(P.mgm);
-            float a = sum_terms(
-                (float)P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1],
-                [&](int t) {
-                    const int kind = term_kind(t);
-                    return (kind == karam::SK_DIODE ? dg : kind == karam::SK_MOS_GDS ? mgds : mgm)[IX(term_idx(t), b, Bp)];
-                },
-                [](float acc, int t, float g) { return term_neg(t) ? acc + (-g) : acc + g; });
-            if (T.slot_diag[sl]) a = a + (float)gmin;
-            if (d & 1) reinterpret_cast<float*>(P.Al)[IX(d >> 1, b, Bp)] = a;
-            else reinterpret_cast<float*>(P.pool)[IX(d >> 1, b, Bp)] = a;
+            float a = hub_sum<float>((float)P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1], lane,
+                                     w, on, slot_src, (size_t)b, Bp, lds);
+            if (w == 0 && on) {
+                if (T.slot_diag[sl]) a = a + (float)gmin_tab[min(P.st[b].step, gmin_last)];
+                if (d & 1) reinterpret_cast<float*>(P.Al)[IX(d >> 1, b, Bp)] = a;
+                else reinterpret_cast<float*>(P.pool)[IX(d >> 1, b, Bp)] = a;
+            }
         } else {
-            double a = sum_terms(
-                P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1],
-                [&](int t) {
-                    const int kind = term_kind(t);
-                    return (kind == karam::SK_DIODE ? P.dg : kind == karam::SK_MOS_GDS ? P.mgds : P.mgm)[IX(term_idx(t), b, Bp)];
-                },
-                [](double acc, int t, double g) { return term_neg(t) ? acc + (-g) : acc + g; });
-            if (T.slot_diag[sl]) a = a + gmin;
-            if (d & 1) P.Al[IX(d >> 1, b, Bp)] = a;
-            else P.pool[IX(d >> 1, b, Bp)] = a;
+            double a = hub_sum<double>(P.glin[IX(sl, b, Bp)], T.nl_term, T.nl_ptr[sl], T.nl_ptr[sl + 1], lane, w,
+                                       on, slot_src, (size_t)b, Bp, lds);
+            if (w == 0 && on) {
+                if (T.slot_diag[sl]) a = a + gmin_tab[min(P.st[b].step, gmin_last)];
+                if (d & 1) P.Al[IX(d >> 1, b, Bp)] = a;
+                else P.pool[IX(d >> 1, b, Bp)] = a;
+            }
         }
     } else if (item < T.n_hub_slot + T.n_hub_row && !P.p32) {
         const int i = T.hub_row[item - T.n_hub_slot];
-        const double r = sum_terms(
-            P.rhslin[IX(i, b, Bp)], T.rhs_nl_term, T.rhs_nl_ptr[i], T.rhs_nl_ptr[i + 1],
-            [&](int t) { return (term_kind(t) == karam::TK_DIODE ? P.dieq : P.mieq)[IX(term_idx(t), b, Bp)]; },
-            [](double acc, int t, double x) { return term_neg(t) ? acc + (-x) : acc + x; });
-        const int d = T.node_dst[i];
-        if (d & 1) P.rhsl[IX(d >> 1, b, Bp)] = r;
-        else P.pool[IX(d >> 1, b, Bp)] = r;
+        const double r = hub_sum<double>(P.rhslin[IX(i, b, Bp)], T.rhs_nl_term, T.rhs_nl_ptr[i],
+                                         T.rhs_nl_ptr[i + 1], lane, w, on, row_src, (size_t)b, Bp, lds);
+        if (w == 0 && on) {
+            const int d = T.node_dst[i];
+            if (d & 1) P.rhsl[IX(d >> 1, b, Bp)] = r;
+            else P.pool[IX(d >> 1, b, Bp)] = r;
+        }
     }
 }
 

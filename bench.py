@@ -21,6 +21,10 @@ HBM.  Prints ONE JSON line:
 
     python bench.py [--steps K] [--warmup W] [--instances B] [--cpu-seconds S]
     python bench.py --copy-probe      # plain device copy for counter scaling
+
+Other workloads (--workload): logic (random CMOS logic, DC and transient),
+mixed (uneven circuit sizes), ac (AC sweeps of the mesh with capacitors),
+noise (noise analysis of the same circuit).
 """
 from __future__ import annotations
 
@@ -338,13 +342,12 @@ def logic_workload(a):
     return 0
 
 
-def ac_workload(a):
-    """AC small-signal sweeps of the benchmark mesh (64 x 64 R-diode mesh with
-    a 2 pF capacitor at every node): DC operating points of the Monte-Carlo
-    instances, then a.ac_points log-spaced frequencies from 1 Hz to 10 GHz,
-    excited at node 1 as solveAC excites it.  One frequency point of one
-    instance is one complex sparse factorisation and solve."""
-    from karam_amd import _abi, netlists, nr
+def small_signal_batch(a):
+    """The AC / noise workloads' circuit: the benchmark mesh (R-diode mesh with
+    a 2 pF capacitor at every node), its Monte-Carlo instances and their DC
+    operating points, and a batch on the transient topology seeded with
+    them."""
+    from karam_amd import netlists, nr
     from karam_amd.netlists import _f64, _i32
     B = a.instances if a.instances != 4096 else 1024
     nl = netlists.rd_mesh(a.side, seed=1)
@@ -362,6 +365,16 @@ def ac_workload(a):
     bt = nr.Batch(ctx, Tt, B)
     bt.set_fields(f)
     bt.set_seed(V)
+    return nl, f, V, Tt, bt, B, n
+
+
+def ac_workload(a):
+    """AC small-signal sweeps of the benchmark mesh (64 x 64 R-diode mesh with
+    a 2 pF capacitor at every node): DC operating points of the Monte-Carlo
+    instances, then a.ac_points log-spaced frequencies from 1 Hz to 10 GHz,
+    excited at node 1 as solveAC excites it.  One frequency point of one
+    instance is one complex sparse factorisation and solve."""
+    nl, f, V, Tt, bt, B, n = small_signal_batch(a)
     probes = [1, 2, n // 2, n]
     kw = dict(f_start=1.0, f_stop=10e9, input_node=1, amplitude=1.0, probes=probes)
     for _ in range(a.warmup):
@@ -397,6 +410,56 @@ def ac_workload(a):
         out["parity_sample"] = {"instances": m, "max_err_over_tol": float(err.max()),
                                 "tolerance": "1e-9 rel + 1e-12 x max |H|"}
         log(f"ac cpu baseline {m * a.ac_points / t_cpu:.1f} points/s; parity {out['parity_sample']}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def noise_workload(a):
+    """Noise analysis of the benchmark mesh (the AC workload's circuit and
+    operating points): output noise PSD at the middle node over a.ac_points
+    log-spaced frequencies from 1 Hz to 10 GHz, and its integral.  Sources:
+    every resistor's thermal noise and every diode's shot noise.  One
+    frequency point of one instance is one complex sparse factorisation of
+    Y^T, one solve and the sum over the sources."""
+    nl, f, V, Tt, bt, B, n = small_signal_batch(a)
+    out_node = n // 2 + a.side // 2
+    kw = dict(f_start=1.0, f_stop=10e9, output_node=out_node, temp_K=300.15)
+    for _ in range(a.warmup):
+        bt.noise(n_points=2, **kw)
+    t0 = time.perf_counter()
+    freqs, psd, vrms, st = bt.noise(n_points=a.ac_points, record_timing=2, **kw)
+    wall = time.perf_counter() - t0
+    kst = bt.kernel_stats()
+    points = B * a.ac_points
+    n_src = len(nl.res_n1) + len(nl.dio_anode) + len(nl.mos_d)
+    log(f"noise: {points} instance-frequency points in {wall:.2f} s, singular {(st != 0).sum()}, "
+        f"vrms median {np.median(vrms):.3e} V")
+    out = {
+        "metric": "noise instance-frequency points/sec", "value": points / wall, "unit": "points/s", "n_gpus": 1,
+        "steps": a.ac_points, "warmup": a.warmup, "ms_per_step": wall * 1e3 / a.ac_points, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "complex f64",
+        "data": "synthetic: seeded R-diode mesh with 2 pF per node, per-instance R and IS sigma 5 %",
+        "config": {"workload": f"rdmesh{a.side} + C, noise at node {out_node}, 1 Hz - 10 GHz, {a.ac_points} points",
+                   "instances": B, "nodes": n, "noise_sources": n_src, "nnz_f": Tt.info["nnz_f"]},
+        "kernels": kernel_table(kst), "roofline": None, "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        from oracle import oracle as O
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        m = min(B, threads)
+        To = O.Topology(nl, transient=True)
+        t0 = time.perf_counter()
+        fo, po, vo, so = O.solve_noise_batch(nl, To, {k: v[:m] for k, v in f.items()}, m, V[:m],
+                                             n_points=a.ac_points, mode=O.NOISE_ADJOINT, threads=threads, **kw)
+        t_cpu = time.perf_counter() - t0
+        err = np.abs(psd[:m] - po) / (1e-9 * np.abs(po))
+        out["cpu_baseline"] = {"value": m * a.ac_points / t_cpu, "unit": "points/s", "cores": threads, "kind": "port",
+                               "sample": f"first {m} of {B} instances, all {a.ac_points} points, oracle (one "
+                                         f"transposed complex multifrontal solve per point, as the GPU), one "
+                                         f"instance per thread; the reference's formulation solves once per "
+                                         f"source ({n_src} per point)"}
+        out["parity_sample"] = {"instances": m, "max_err_over_tol": float(err.max()), "tolerance": "1e-9 rel"}
+        log(f"noise cpu baseline {m * a.ac_points / t_cpu:.1f} points/s; parity {out['parity_sample']}")
     print(json.dumps(out), flush=True)
     return 0
 
@@ -500,7 +563,7 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
-    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac"], default="rdmesh64")
+    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac", "noise"], default="rdmesh64")
     ap.add_argument("--ac-points", type=int, default=50)
     ap.add_argument("--mixed-instances", type=int, default=10000)
     ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
@@ -523,6 +586,8 @@ def main():
         return mixed_workload(a)
     if a.workload == "ac":
         return ac_workload(a)
+    if a.workload == "noise":
+        return noise_workload(a)
 
     from karam_amd import _abi, netlists, nr
 

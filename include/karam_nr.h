@@ -296,7 +296,7 @@ NR_API int nr_topology_build_ex(nr_ctx* ctx, const nr_netlist* nl, int32_t kind,
 
 enum {
     NR_TRAN_START_ZERO = 0,  /* zero voltages, history and arbiter state (the
-                                reference's start, D11)                     */
+                                reference's start)                          */
     NR_TRAN_START_SEED = 1   /* the batch seed, e.g. a DC operating point:
                                 capacitor history at its branch voltages
                                 with zero current, arbiter at that state    */
@@ -408,7 +408,7 @@ NR_API int nr_batch_get_tran_stats(const nr_batch* batch, nr_tran_stats* out, in
  * conductances as buildAdmittanceMatrix takes them (:186-196: the derivative
  * of the clamped diode current, at least 1e-15 S), MOSFET gm / gds of the
  * instance's devices (the DC Jacobian's; the reference's own AC builder uses
- * fixed NMOS parameters, D7), the 1e-12 diagonal, and the excitation as
+ * fixed NMOS parameters), the 1e-12 diagonal, and the excitation as
  * solveAC applies it: 1e6 S from the input node to ground with 1e6 x
  * amplitude A injected.  Frequencies: n_points log-spaced from
  * max(f_start, 1e-3) to max(f_stop, f_start + 1) Hz, as solveAC spaces them.
@@ -435,6 +435,40 @@ NR_API int nr_batch_get_ac(const nr_batch* batch, double* out, int64_t count);
 NR_API int nr_batch_get_ac_freqs(const nr_batch* batch, double* out, int64_t count);
 NR_API int nr_batch_get_ac_status(const nr_batch* batch, int32_t* out, int64_t count);
 
+/* ---- noise analysis ------------------------------------------------------ */
+/* ACSolver::solveNoise (solvers/ac_solver.h:524-595) for every instance at
+ * its operating point: the output noise PSD S_out(f) = sum_k |H_k|^2 S_k
+ * (V^2/Hz) at the output node and sqrt of its trapezoid integral over the
+ * frequency points (V rms).  Y(jw) is the AC sweep's without the excitation.
+ * Sources in computeNoiseSources order (:400-504): resistors 4kT/R, diodes
+ * 2q|I| of the clamped diode current at the operating point, MOSFETs the
+ * channel's 4kT (2/3) gm (gm of the instance's device, as in the AC sweep;
+ * the reference takes fixed NMOS parameters).  Flicker noise needs KF /
+ * C_ox, which the reference defaults to 0 and the netlist does not carry.
+ * The reference solves Y V = e_pos - e_neg once per source; here each
+ * frequency is one complex factorisation of Y^T and one solve Y^T z = e_out
+ * for every instance, and H_k = z[pos] - z[neg] (the same transfer function,
+ * rounded differently).  input_node is accepted as solveNoise accepts it and,
+ * as there, does not change the result. */
+typedef struct nr_noise_opts {
+    double f_start, f_stop;  /* Hz, spaced as in nr_ac_opts                 */
+    int32_t n_points;        /* frequency points (>= 1)                     */
+    int32_t output_node;     /* 1-based observed node; 0: zero noise        */
+    double temp_K;           /* temperature of the thermal sources (300.15) */
+    int32_t input_node;      /* 1-based, unused (see above)                 */
+    int32_t record_timing;   /* as in nr_solve_opts                         */
+} nr_noise_opts;
+NR_API void nr_noise_opts_default(nr_noise_opts* opts);
+NR_API int nr_batch_noise(nr_batch* batch, const nr_noise_opts* opts, nr_result* result);
+/* Results of the last noise run: psd[b][f], count = B * n_points; the
+ * integrated RMS noise per instance, count = B; the frequencies, count =
+ * n_points; per-instance status (NR_STATUS_SINGULAR if a pivot vanished at
+ * some frequency), count = B. */
+NR_API int nr_batch_get_noise(const nr_batch* batch, double* psd, int64_t count);
+NR_API int nr_batch_get_noise_vrms(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_noise_freqs(const nr_batch* batch, double* out, int64_t count);
+NR_API int nr_batch_get_noise_status(const nr_batch* batch, int32_t* out, int64_t count);
+
 /* Per-instance results.  out arrays are caller-owned host buffers. */
 NR_API int nr_batch_get_voltages(const nr_batch* batch, double* out, int64_t count);
 enum {
@@ -444,7 +478,7 @@ enum {
 };
 typedef struct nr_instance_stats {
     int32_t iterations;          /* true NR iteration count                  */
-    int32_t ref_iterations;      /* reference SolverStats accounting (D15)   */
+    int32_t ref_iterations;      /* reference SolverStats accounting         */
     int32_t status;              /* NR_STATUS_*                              */
     int32_t gmin_steps_done;
     double kcl;                  /* final max |KCL| (A), FP64                */
@@ -492,7 +526,7 @@ NR_API int nr_batch_get_kernel_stats(const nr_batch* batch, nr_kernel_stat* out,
 
 /* ---- learned seeder behind the NeuralSeeder seam ------------------------- */
 /* solvers/neural_seeder.h:45-220 exposes recordSolution / predictSeed /
- * hasHistory / reset around an exponential-average blender (D20).  Here the
+ * hasHistory / reset around an exponential-average blender.  Here the
  * same seam holds a multilayer perceptron: n_features -> n_hidden layers of
  * `hidden` units with GELU -> n_out node voltages, BF16 weights and
  * activations in the matrix products with FP32 accumulation, FP32 master

@@ -187,6 +187,13 @@ class NrSpiceInfo(C.Structure):
     ]
 
 
+class NrNoiseOpts(C.Structure):
+    _fields_ = [
+        ("f_start", C.c_double), ("f_stop", C.c_double), ("n_points", C.c_int32), ("output_node", C.c_int32),
+        ("temp_K", C.c_double), ("input_node", C.c_int32), ("record_timing", C.c_int32),
+    ]
+
+
 class NrAcOpts(C.Structure):
     _fields_ = [
         ("f_start", C.c_double), ("f_stop", C.c_double), ("n_points", C.c_int32), ("input_node", C.c_int32),

@@ -1799,7 +1799,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     const int n = T.n;
     InstState st = P.st[b];
     if (st.singular) {
-        // singular Jacobian: abort the GMIN step and the solve (D15 accounting)
+        // singular Jacobian: abort the GMIN step and the solve (the reference's count)
         if (threadIdx.x == 0) {
             st.total += st.it + 1;
             st.ref += 2 * (st.it + 1);
@@ -2031,6 +2031,93 @@ __global__ void k_ac_out(DevTopo T, DevBatch Q, int nact, int f, int n_freq, int
     o[0] = x.re;
     o[1] = x.im;
     if (p == 0 && Q.st[b].singular) status[b] = NR_STATUS_SINGULAR;
+}
+
+// Noise of frequency point f: after the solve of Y^T z = e_out (Q.xs), the
+// output noise PSD sum_k |z[pos_k] - z[neg_k]|^2 S_k over the sources in
+// computeNoiseSources order (solvers/ac_solver.h:400-504): src[k] = (kind,
+// device, pos, neg) with kind 0 a resistor (netlist index; 4kT/R), 1 a diode
+// (SoA index; 2q|I| of the clamped diode current at the operating point), 2 a
+// MOSFET (SoA index; 4kT (2/3) gm when gm > 0).  A source whose PSD is not
+// positive is left out, as the reference leaves it out of its list
+// (physics/noise_models.h); its term is stored as +0, which leaves the sum of
+// non-negative terms unchanged.  One block per 64 instances: its sixteen
+// wavefronts evaluate 256 terms at a time into LDS, and wavefront 0 adds them
+// in source order.
+constexpr double NOISE_KB = 1.380649e-23, NOISE_QE = 1.602176634e-19;
+__device__ __forceinline__ double noise_term(const DevTopo& T, const DevBatch& P, const DevBatch& Q, int b, int4 e,
+                                             double temp_K) {
+#pragma clang fp contract(off)
+    const int n = T.n, Bp = P.Bp;
+    auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
+    auto Z = [&](int node) -> cdouble {
+        return (node > 0 && node <= n) ? vp<cdouble>(Q.xs)[IX(T.iperm[node - 1], b, Bp)] : cdouble(0.0, 0.0);
+    };
+    double s;
+    if (e.x == 0) {
+        const double R = P.R[(size_t)b * T.nR + e.y];
+        s = R <= 0.0 ? 0.0 : 4.0 * NOISE_KB * temp_K / R;
+    } else if (e.x == 1) {
+        const double I = diode_i_clamped(V(e.z) - V(e.w), P.Is[(size_t)b * T.nD + e.y], T.d_n[e.y], T.d_vtraw[e.y]);
+        s = 2.0 * NOISE_QE * fabs(I);
+    } else {
+        const double gm = P.mgm[IX(e.y, b, Bp)];
+        s = gm <= 0.0 ? 0.0 : 4.0 * NOISE_KB * temp_K * (2.0 / 3.0) * gm;
+    }
+    if (!(s > 0.0)) return 0.0;
+    const cdouble zp = Z(e.z), zn = Z(e.w);
+    const double hr = zp.re - zn.re, hi = zp.im - zn.im;
+    return (hr * hr + hi * hi) * s;
+}
+
+__global__ __launch_bounds__(HUB_SUM_THREADS) void k_noise_sum(DevTopo T, DevBatch P, DevBatch Q, int nact,
+                                                               const int4* __restrict__ src, int n_src, double temp_K,
+                                                               int f, int n_freq, double* __restrict__ psd,
+                                                               int* __restrict__ status) {
+#pragma clang fp contract(off)
+    __shared__ double buf[HUB_SUM_CHUNK][64];
+    constexpr int PER = HUB_SUM_CHUNK / (HUB_SUM_THREADS / 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int b0 = blockIdx.x * 64, b = b0 + lane;
+    const bool on = b < nact;
+    const int bb = on ? b : b0;
+    double total = 0.0;
+    for (int base = 0; base < n_src; base += HUB_SUM_CHUNK) {
+        const int cnt = min(HUB_SUM_CHUNK, n_src - base);
+        double x[PER];
+#pragma unroll
+        for (int u = 0; u < PER; ++u) {
+            const int q = w * PER + u;
+            x[u] = q < cnt ? noise_term(T, P, Q, bb, src[base + q], temp_K) : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < PER; ++u) buf[w * PER + u][lane] = x[u];
+        __syncthreads();
+        if (w == 0)
+            for (int q = 0; q < cnt; ++q) total = total + buf[q][lane];
+        __syncthreads();
+    }
+    if (w == 0 && on) {
+        psd[(size_t)b * n_freq + f] = total;
+        if (Q.st[b].singular) status[b] = NR_STATUS_SINGULAR;
+    }
+}
+
+// sqrt of the trapezoid integral of each instance's PSD over the frequency
+// points (solvers/ac_solver.h:585-592)
+__global__ void k_noise_integrate(const double* __restrict__ freqs, const double* __restrict__ psd, int n_freq,
+                                  int nact, double* __restrict__ vrms) {
+#pragma clang fp contract(off)
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nact) return;
+    const double* p = psd + (size_t)b * n_freq;
+    double integrated = 0.0;
+    for (int i = 1; i < n_freq; ++i) {
+        const double df = freqs[i] - freqs[i - 1];
+        const double avg = 0.5 * (p[i] + p[i - 1]);
+        integrated += avg * df;
+    }
+    vrms[b] = sqrt(integrated > 0.0 ? integrated : 0.0);
 }
 
 __global__ __launch_bounds__(256) void k_final_kcl(DevTopo T, DevBatch P) {

@@ -9,7 +9,7 @@
 //   symbolicFactorize           math/sparse_lu.cpp:14-128
 // Min-Fill is computed incrementally (fill counts updated only where an
 // elimination can change them) with the reference's selection rule — min
-// fill, ties to the smallest id, duplicate ids as coded (D4) — so it yields
+// fill, ties to the smallest id, duplicate ids as coded — so it yields
 // the same ordering as the reference's full rescan.  The fill pattern comes
 // from the elimination tree (column structure = own lower entries united with
 // the children's structures), which equals the reference's simulated graph
@@ -91,7 +91,7 @@ void tensorize(const nr_netlist* nl, SoA& s) {
                 if (pm) { kp = 100e-6; vth = -0.7; }
             }
         }
-        if (nl->mos_kp) kp = nl->mos_kp[i];     // D14: per-device MC override
+        if (nl->mos_kp) kp = nl->mos_kp[i];     // per-device MC override
         if (nl->mos_vto) vth = nl->mos_vto[i];
         s.m_src.push_back(i);
         s.m_d.push_back(nl->mos_d[i]); s.m_g.push_back(nl->mos_g[i]);
@@ -512,7 +512,7 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool tra
         t.treewidth = maxClique > 0 ? (int32_t)maxClique - 1 : 0;
     }
 
-    // ---- D3/D4 projection onto matrix rows -------------------------------------
+    // ---- projection of the ordering onto matrix rows ----------------------------
     {
         std::vector<char> seen(n + 1, 0);
         t.perm.clear();

@@ -188,7 +188,7 @@ def inverter_chain(stages: int = 16, vdd: float = 1.8, cload: float = 10e-15,
                    pulse=(0.0, 1.8, 20e-9, 1e-9, 1e-9, 1.0, 0.0)) -> Netlist:
     """CMOS inverter chain: VDD = node 1, IN = node 2, OUT_k = node 2 + k.
     NMOS W/L = 2/1, PMOS W/L = 4/1, a load capacitor per output; V_IN is a
-    PULSE source whose DC value is its v1 (D8)."""
+    PULSE source whose DC value is its v1."""
     n = 2 + stages
     md, mg, ms, mb, mw, ml, mm = [], [], [], [], [], [], []
     for k in range(1, stages + 1):

@@ -47,6 +47,8 @@ EXPORTED = [
     "nr_spice_parse", "nr_spice_free", "nr_spice_get_netlist", "nr_spice_get_info", "nr_spice_node_name",
     "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
     "nr_ac_opts_default", "nr_batch_ac", "nr_batch_get_ac", "nr_batch_get_ac_freqs", "nr_batch_get_ac_status",
+    "nr_noise_opts_default", "nr_batch_noise", "nr_batch_get_noise", "nr_batch_get_noise_vrms",
+    "nr_batch_get_noise_freqs", "nr_batch_get_noise_status",
 ]
 
 
@@ -79,6 +81,13 @@ def lib():
         L.nr_batch_get_ac.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_ac_freqs.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_ac_status.argtypes = [_vp, C.POINTER(C.c_int32), C.c_int64]
+        L.nr_noise_opts_default.argtypes = [C.POINTER(_abi.NrNoiseOpts)]
+        L.nr_noise_opts_default.restype = None
+        L.nr_batch_noise.argtypes = [_vp, C.POINTER(_abi.NrNoiseOpts), C.POINTER(_abi.NrResult)]
+        L.nr_batch_get_noise.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_noise_vrms.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_noise_freqs.argtypes = [_vp, _f64p, C.c_int64]
+        L.nr_batch_get_noise_status.argtypes = [_vp, C.POINTER(C.c_int32), C.c_int64]
         L.nr_spice_parse.argtypes = [C.c_char_p, C.c_int64, C.c_int32, C.POINTER(_vp)]
         L.nr_spice_free.argtypes = [_vp]
         L.nr_spice_get_netlist.argtypes = [_vp, C.POINTER(_abi.NrNetlist)]
@@ -363,6 +372,29 @@ class Batch:
         _check(lib().nr_batch_get_ac_status(self.h, st.ctypes.data_as(C.POINTER(C.c_int32)), self.B), self.ctx)
         self._ac_result = {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
         return freqs, out[..., 0] + 1j * out[..., 1], st
+
+    def noise(self, output_node, f_start=1.0, f_stop=1e9, n_points=50, temp_K=300.15, input_node=0,
+              record_timing=0):
+        """Noise analysis of every instance at its operating point (the seed);
+        returns (freqs [F], output noise PSD [B, F] in V^2/Hz, integrated RMS
+        noise [B] in V, status [B])."""
+        o = _abi.NrNoiseOpts()
+        lib().nr_noise_opts_default(C.byref(o))
+        o.f_start, o.f_stop, o.n_points = float(f_start), float(f_stop), int(n_points)
+        o.output_node, o.temp_K, o.input_node = int(output_node), float(temp_K), int(input_node)
+        o.record_timing = record_timing
+        r = _abi.NrResult()
+        _check(lib().nr_batch_noise(self.h, C.byref(o), C.byref(r)), self.ctx)
+        psd = np.zeros((self.B, int(n_points)))
+        _check(lib().nr_batch_get_noise(self.h, _p64(psd), psd.size), self.ctx)
+        vrms = np.zeros(self.B)
+        _check(lib().nr_batch_get_noise_vrms(self.h, _p64(vrms), self.B), self.ctx)
+        freqs = np.zeros(int(n_points))
+        _check(lib().nr_batch_get_noise_freqs(self.h, _p64(freqs), freqs.size), self.ctx)
+        st = np.zeros(self.B, dtype=np.int32)
+        _check(lib().nr_batch_get_noise_status(self.h, st.ctypes.data_as(C.POINTER(C.c_int32)), self.B), self.ctx)
+        self._noise_result = {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+        return freqs, psd, vrms, st
 
     def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
         """Choose the large-front launch shape per front-order bucket by timing

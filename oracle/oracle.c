@@ -2,9 +2,8 @@
  * oracle.c — CPU FP64 restatement of the reference NR DC path.
  *
  * TEST INFRASTRUCTURE ONLY (the checker).  Built without fast-math.  Each
- * function follows the reference file:line named in its comment; deviations
- * from literal reference behaviour carry the defect id of the hot-path
- * survey's register (D3, D4, D7, D14, D15).
+ * function follows the reference file:line named in its comment, and its
+ * comment states where it departs from the reference's literal behaviour.
  */
 #include "oracle.h"
 
@@ -112,7 +111,7 @@ orc_soa* orc_tensorize(const nr_netlist* nl) {
     }
     free(items);
     /* MOSFETs: bucket by modelName; parameters from the card (:424-441 via
-     * collectFromBlock :407-420); no cards at all -> NMOS defaults (D9). */
+     * collectFromBlock :407-420); no cards at all -> NMOS defaults. */
     s->nM = nl->n_mos;
     items = (name_item*)xmalloc(sizeof(name_item) * (size_t)(s->nM + 1));
     for (int32_t i = 0; i < s->nM; ++i) {
@@ -143,7 +142,7 @@ orc_soa* orc_tensorize(const nr_netlist* nl) {
                 if (pm) { kp = 100e-6; vth = -0.7; }
             }
         }
-        /* D14 extension: per-device Monte-Carlo KP / VTO overrides */
+        /* per-device Monte-Carlo KP / VTO overrides (an extension) */
         if (nl->mos_kp) kp = nl->mos_kp[i];
         if (nl->mos_vto) vth = nl->mos_vto[i];
         s->m_src[p] = i;
@@ -555,7 +554,7 @@ static int cmp_i32(const void* a, const void* b) {
  * :14-141), restated literally: every step scans all remaining variables in
  * ascending id order, fill = number of non-adjacent neighbour pairs, argmin
  * with ties to the smallest id; branch-current ids start at numCircuitNodes
- * and may collide with a node id (D4, kept as coded). */
+ * and may collide with a node id (kept as coded). */
 static void min_fill(const nr_netlist* nl, const orc_soa* s, orc_topo* t) {
     const int32_t ncn = nl->num_nodes;
     const int32_t nbranch = nl->n_vsrc;        /* + inductors (none in this ABI) */
@@ -619,7 +618,7 @@ static void min_fill(const nr_netlist* nl, const orc_soa* s, orc_topo* t) {
     eg_free(&g);
 }
 
-/* D3/D4 projection of the Min-Fill ordering onto matrix rows: keep the first
+/* Projection of the Min-Fill ordering onto matrix rows: keep the first
  * occurrence of each node id 1..n, drop ids above n, row = id - 1; rows that
  * no device touches (none in practice) are appended in ascending order. */
 static void project_ordering(orc_topo* t) {
@@ -1314,7 +1313,7 @@ void orc_apply_pn_limits(const nr_netlist* nl, const orc_topo* t,
 
 /* ------------------------------------------------------------------------ */
 /* a19: calculatePhysicalResiduals (physics/circuitsim.h:1250-1671), DC     */
-/* terms, netlist order.  D7 fix: the MOSFET term uses the SoA parameters   */
+/* terms, netlist order.  The MOSFET term uses the SoA parameters          */
 /* and the generic-kernel current (same equations as the stamp).            */
 /* ------------------------------------------------------------------------ */
 static double diode_i_clamped(double vd, double Is, double N, double Vt) {
@@ -1464,7 +1463,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
             if (rc != NR_OK) {
                 final_kcl = 1e9;
                 singular = 1; sing_step = 1;
-                ref_total += it + 1;           /* D15: counted here and below */
+                ref_total += it + 1;           /* the reference's count: here and below */
                 break;
             }
             for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
@@ -1535,7 +1534,7 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
 
 /* ------------------------------------------------------------------------ */
 /* a16: fixed-step backward-Euler transient, stepTransient restated          */
-/* (solvers/circuitsim_solver.cpp:559-780; D10-D13)                          */
+/* (solvers/circuitsim_solver.cpp:559-780)                                   */
 /* ------------------------------------------------------------------------ */
 /* PULSE value at time `time` (physics/circuitsim.h:2155-2170);
  * p = v1 v2 td tr tf pw per */
@@ -1742,7 +1741,7 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
-    /* start state (D11): zero voltages, history and arbiter (the reference's
+    /* start state: zero voltages, history and arbiter (the reference's
      * IntegratorArbiter::initialize, :135-173, and zero-filled history), or
      * the caller's state with the history and arbiter at it */
     if (o.start) memcpy(v, v_io, nb);
@@ -1951,8 +1950,11 @@ static inline double c_mag(cplx a) {
     return x >= y ? x : y;
 }
 
-/* orc_lu_multifrontal on complex values (same operation order) */
-static int lu_multifrontal_c(const orc_topo* t, const cplx* a_vals, const cplx* rhs, cplx* x) {
+/* orc_lu_multifrontal on complex values (same operation order).  With
+ * transpose = 1 it factors and solves A^T instead: each A entry (i, j) is
+ * placed at the front position of (j, i), which lies in the same front
+ * because the filled pattern is symmetric. */
+static int lu_multifrontal_c(const orc_topo* t, const cplx* a_vals, const cplx* rhs, cplx* x, int transpose) {
     const int32_t n = t->n, ns = t->ns;
     int64_t* u_off = (int64_t*)xmalloc(8 * (size_t)ns);
     int64_t* cb_off = (int64_t*)xmalloc(8 * (size_t)ns);
@@ -1981,7 +1983,7 @@ static int lu_multifrontal_c(const orc_topo* t, const cplx* a_vals, const cplx* 
         for (int32_t e = 0; e < r * r; ++e) F[e] = zero;
         for (int32_t e = 0; e < r; ++e) fr[e] = zero;
         for (int32_t q = t->sn_a_ptr[J]; q < t->sn_a_ptr[J + 1]; ++q) {
-            cplx* d = &F[t->sn_a_li[q] * r + t->sn_a_lj[q]];
+            cplx* d = transpose ? &F[t->sn_a_lj[q] * r + t->sn_a_li[q]] : &F[t->sn_a_li[q] * r + t->sn_a_lj[q]];
             *d = c_add(*d, a_vals[t->sn_a_slot[q]]);
         }
         for (int32_t k = 0; k < c; ++k) fr[k] = c_add(fr[k], rhs[t->perm[f + k]]);
@@ -2097,7 +2099,7 @@ int orc_solve_ac(const nr_netlist* nl, const orc_topo* t, const double* v_op, co
             rhs[o->input_node - 1].re = 0.0 + 1e6 * o->amplitude;
             rhs[o->input_node - 1].im = 0.0 + 0.0;
         }
-        if (lu_multifrontal_c(t, A, rhs, x) != NR_OK) *status = NR_STATUS_SINGULAR;
+        if (lu_multifrontal_c(t, A, rhs, x, 0) != NR_OK) *status = NR_STATUS_SINGULAR;
         for (int32_t p = 0; p < n_probe; ++p) {
             out[((size_t)f * n_probe + p) * 2] = x[probe[p] - 1].re;
             out[((size_t)f * n_probe + p) * 2 + 1] = x[probe[p] - 1].im;
@@ -2131,6 +2133,158 @@ int orc_solve_ac_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const
         }
         bad |= orc_solve_ac(&inst, t, v_op + (size_t)b * n, o, n_probe, probe,
                             out + (size_t)b * o->n_points * n_probe * 2, &status[b]) != NR_OK;
+    }
+    return bad ? NR_ERR_INVALID : NR_OK;
+}
+
+/* ---- noise analysis (ACSolver::solveNoise, solvers/ac_solver.h:524-583) ----
+ * Y(jw) as the AC sweep builds it without an excitation (the reference's
+ * noise path calls buildAdmittanceMatrix alone, :559-561).  Noise current
+ * sources in computeNoiseSources order (:400-504): every resistor, 4kT/R
+ * between its terminals; every diode, 2q|I(vd)| of the clamped diode current
+ * at the operating point, anode to cathode; every MOSFET, the channel's
+ * 4kT (2/3) gm drain to source when gm > 0 (physics/noise_models.h).  The
+ * MOSFET's gm is that of the device the DC solve used, as in the AC sweep.
+ * Flicker noise needs KF and C_ox, which default to 0 (netlist/circuit.h:500-
+ * 502) and have no netlist field here, so it never applies; BJTs are not part
+ * of the netlist.
+ *   S_out(f) = sum over sources of |H_k|^2 S_k, added in source order, with
+ *   |H|^2 = re re + im im.
+ * mode ORC_NOISE_PER_SOURCE restates the reference: one solve of Y V =
+ * e_pos - e_neg per source, H_k = V[out] (:565-581).  mode ORC_NOISE_ADJOINT
+ * is the GPU's formulation: one solve of Y^T z = e_out per frequency, H_k =
+ * z[pos] - z[neg] (the same number up to rounding, since V[out] = e_out^T
+ * Y^-1 (e_pos - e_neg)).  The integral is the trapezoid over the frequency
+ * points (:585-592). */
+static const double NOISE_KB = 1.380649e-23, NOISE_QE = 1.602176634e-19;
+
+typedef struct { int32_t pos, neg; double psd; } noise_src;
+
+static int32_t noise_sources(const nr_netlist* nl, const orc_soa* s, const double* v_op, double temp_K,
+                             noise_src* out) {
+    const int32_t n = nl->num_nodes;
+    int32_t k = 0;
+    for (int32_t i = 0; i < nl->n_res; ++i) {
+        const double R = nl->res_ohms[i];
+        const double psd = R <= 0.0 ? 0.0 : 4.0 * NOISE_KB * temp_K / R;
+        if (psd > 0.0) { out[k].pos = nl->res_n1[i]; out[k].neg = nl->res_n2[i]; out[k].psd = psd; ++k; }
+    }
+    for (int32_t i = 0; i < nl->n_diode; ++i) {
+        const int32_t a = nl->dio_anode[i], c = nl->dio_cathode[i];
+        const double I = diode_i_clamped(volt(v_op, n, a) - volt(v_op, n, c), nl->dio_is[i], nl->dio_n[i],
+                                         nl->dio_vt[i]);
+        const double psd = 2.0 * NOISE_QE * fabs(I);
+        if (psd > 0.0) { out[k].pos = a; out[k].neg = c; out[k].psd = psd; ++k; }
+    }
+    int32_t* soa_of = (int32_t*)xmalloc(4 * (size_t)(s->nM + 1));
+    for (int32_t p = 0; p < s->nM; ++p) soa_of[s->m_src[p]] = p;
+    for (int32_t i = 0; i < nl->n_mos; ++i) {
+        const int32_t p = soa_of[i];
+        double vgs, vds, ids, gm, gds;
+        mos_one(volt(v_op, n, s->mg[p]), volt(v_op, n, s->md[p]), volt(v_op, n, s->ms[p]), s->mW[p], s->mL[p],
+                s->mKp[p], s->mVth[p], s->mLam[p], s->mP[p], &vgs, &vds, &ids, &gm, &gds);
+        const double psd = gm <= 0.0 ? 0.0 : 4.0 * NOISE_KB * temp_K * (2.0 / 3.0) * gm;
+        if (psd > 0.0) { out[k].pos = s->md[p]; out[k].neg = s->ms[p]; out[k].psd = psd; ++k; }
+    }
+    free(soa_of);
+    return k;
+}
+
+int orc_solve_noise(const nr_netlist* nl, const orc_topo* t, const double* v_op, const orc_noise_opts* o,
+                    double* psd_out, double* vrms, int32_t* status) {
+    const int32_t n = nl->num_nodes;
+    if (!t || !t->transient || o->n_points <= 0 || o->output_node < 0 || o->output_node > n) return NR_ERR_INVALID;
+    orc_soa* s = orc_tensorize(nl);
+    const int32_t F = o->n_points;
+    orc_ac_opts fo = {o->f_start, o->f_stop, F, 0, 1.0};
+    double* freqs = (double*)xmalloc(sizeof(double) * (size_t)F);
+    orc_ac_freqs(&fo, freqs);
+    double* vre = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* vim = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
+    double* are = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    double* aim = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+    cplx* A = (cplx*)xmalloc(sizeof(cplx) * (size_t)t->nnz_a);
+    cplx* rhs = (cplx*)xmalloc(sizeof(cplx) * (size_t)n);
+    cplx* x = (cplx*)xmalloc(sizeof(cplx) * (size_t)n);
+    noise_src* src = (noise_src*)xmalloc(sizeof(noise_src) * (size_t)(nl->n_res + nl->n_diode + nl->n_mos + 1));
+    const int32_t ns = noise_sources(nl, s, v_op, o->temp_K, src);
+    tran_ctx tc = {1.0, ORC_INT_GEAR1, NULL, NULL, NULL, NULL, NULL, 0};
+    emitter e;
+    memset(&e, 0, sizeof(e));
+    e.vals = vre;
+    e.ac = 1;
+    stamp_all(nl, s, v_op, 1e-12, &e, &tc);
+    int rc = e.nv == t->n_stamps ? NR_OK : NR_ERR_TOPOLOGY;
+    orc_assemble(t, vre, are);
+    *status = NR_STATUS_CONVERGED;
+    const int32_t out = o->output_node;
+    for (int32_t f = 0; f < F && rc == NR_OK; ++f) {
+        const double omega = 2.0 * 3.14159265358979323846 * freqs[f];
+        memset(&e, 0, sizeof(e));
+        e.vals = vim;
+        e.ac = 2;
+        e.omega = omega;
+        stamp_all(nl, s, v_op, 1e-12, &e, &tc);
+        orc_assemble(t, vim, aim);
+        for (int32_t k = 0; k < t->nnz_a; ++k) { A[k].re = are[k]; A[k].im = aim[k]; }
+        double total = 0.0;
+        if (o->mode == ORC_NOISE_ADJOINT) {
+            for (int32_t i = 0; i < n; ++i) { rhs[i].re = 0.0; rhs[i].im = 0.0; }
+            if (out > 0) { rhs[out - 1].re = 0.0 + 1.0; rhs[out - 1].im = 0.0 + 0.0; }
+            if (lu_multifrontal_c(t, A, rhs, x, 1) != NR_OK) *status = NR_STATUS_SINGULAR;
+            for (int32_t k = 0; k < ns; ++k) {
+                const cplx zp = src[k].pos > 0 ? x[src[k].pos - 1] : (cplx){0.0, 0.0};
+                const cplx zn = src[k].neg > 0 ? x[src[k].neg - 1] : (cplx){0.0, 0.0};
+                const double hr = zp.re - zn.re, hi = zp.im - zn.im;
+                total = total + (hr * hr + hi * hi) * src[k].psd;
+            }
+        } else {
+            for (int32_t k = 0; k < ns; ++k) {
+                for (int32_t i = 0; i < n; ++i) { rhs[i].re = 0.0; rhs[i].im = 0.0; }
+                if (src[k].pos > 0) { rhs[src[k].pos - 1].re += 1.0; rhs[src[k].pos - 1].im += 0.0; }
+                if (src[k].neg > 0) { rhs[src[k].neg - 1].re -= 1.0; rhs[src[k].neg - 1].im -= 0.0; }
+                if (lu_multifrontal_c(t, A, rhs, x, 0) != NR_OK) *status = NR_STATUS_SINGULAR;
+                const cplx h = out > 0 ? x[out - 1] : (cplx){0.0, 0.0};
+                total = total + (h.re * h.re + h.im * h.im) * src[k].psd;
+            }
+        }
+        psd_out[f] = total;
+    }
+    double integrated = 0.0;
+    for (int32_t i = 1; i < F; ++i) {
+        const double df = freqs[i] - freqs[i - 1];
+        const double avg = 0.5 * (psd_out[i] + psd_out[i - 1]);
+        integrated += avg * df;
+    }
+    *vrms = sqrt(integrated > 0.0 ? integrated : 0.0);
+    free(freqs); free(vre); free(vim); free(are); free(aim); free(A); free(rhs); free(x); free(src);
+    orc_soa_free(s);
+    return rc;
+}
+
+int orc_solve_noise_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                          const double* v_op, const orc_noise_opts* o, int32_t threads, double* psd_out,
+                          double* vrms, int32_t* status) {
+    if (!t || B <= 0) return NR_ERR_INVALID;
+    const int32_t n = nl->num_nodes;
+    int bad = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad)
+#endif
+    for (int32_t b = 0; b < B; ++b) {
+        nr_netlist inst = *nl;
+        if (fields) {
+            if (fields[NR_FIELD_RES_OHMS]) inst.res_ohms = fields[NR_FIELD_RES_OHMS] + (size_t)b * nl->n_res;
+            if (fields[NR_FIELD_DIO_IS]) inst.dio_is = fields[NR_FIELD_DIO_IS] + (size_t)b * nl->n_diode;
+            if (fields[NR_FIELD_MOS_KP]) inst.mos_kp = fields[NR_FIELD_MOS_KP] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_MOS_VTO]) inst.mos_vto = fields[NR_FIELD_MOS_VTO] + (size_t)b * nl->n_mos;
+            if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
+            if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
+            if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+        }
+        bad |= orc_solve_noise(&inst, t, v_op + (size_t)b * n, o, psd_out + (size_t)b * o->n_points, &vrms[b],
+                               &status[b]) != NR_OK;
     }
     return bad ? NR_ERR_INVALID : NR_OK;
 }

@@ -133,7 +133,7 @@ typedef struct orc_tran_opts {
     double h;             /* step (s); the arbiter's requested step       */
     int32_t steps;        /* fixed control: steps to take                 */
     int32_t max_iter;     /* 50                                           */
-    int32_t start;        /* 0: zero state (D11), 1: v_io holds the state */
+    int32_t start;        /* 0: zero state, 1: v_io holds the state */
     int32_t lu_kind;      /* ORC_LU_* (FP64 path)                         */
     int32_t precision;    /* ORC_PREC_*                                   */
     int32_t method;       /* ORC_INT_* at t = 0                           */
@@ -187,6 +187,23 @@ int orc_solve_ac(const nr_netlist* nl, const orc_topo* t, const double* v_op, co
 int orc_solve_ac_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
                        const double* v_op, const orc_ac_opts* o, int32_t threads, int32_t n_probe,
                        const int32_t* probe, double* out, int32_t* status);
+
+/* --- noise analysis (ACSolver::solveNoise) ------------------------------------ */
+enum { ORC_NOISE_PER_SOURCE = 0, ORC_NOISE_ADJOINT = 1 };
+typedef struct orc_noise_opts {
+    double f_start, f_stop;
+    int32_t n_points;
+    int32_t output_node;  /* 1-based; 0 gives zero noise */
+    double temp_K;
+    int32_t mode;         /* ORC_NOISE_*                 */
+    int32_t pad;
+} orc_noise_opts;
+/* psd_out[f] (V^2/Hz), vrms = sqrt of the trapezoid integral */
+int orc_solve_noise(const nr_netlist* nl, const orc_topo* t, const double* v_op, const orc_noise_opts* o,
+                    double* psd_out, double* vrms, int32_t* status);
+int orc_solve_noise_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const double* const* fields,
+                          const double* v_op, const orc_noise_opts* o, int32_t threads, double* psd_out,
+                          double* vrms, int32_t* status);
 
 int orc_tensorize_export(const nr_netlist* nl, int32_t* d_a, int32_t* d_c, double* d_is, double* d_n,
                          double* d_vt, int32_t* m_d, int32_t* m_g, int32_t* m_s, int32_t* m_b,

@@ -43,6 +43,14 @@ class OrcAcOpts(C.Structure):
                 ("input_node", C.c_int32), ("amplitude", C.c_double)]
 
 
+class OrcNoiseOpts(C.Structure):
+    _fields_ = [("f_start", C.c_double), ("f_stop", C.c_double), ("n_points", C.c_int32),
+                ("output_node", C.c_int32), ("temp_K", C.c_double), ("mode", C.c_int32), ("pad", C.c_int32)]
+
+
+NOISE_PER_SOURCE, NOISE_ADJOINT = 0, 1
+
+
 class OrcDcOut(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
                 ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
@@ -77,6 +85,8 @@ def lib():
         L.orc_ac_freqs.argtypes = [C.POINTER(OrcAcOpts), _f64p]
         L.orc_solve_ac_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32, C.POINTER(_f64p), _f64p,
                                          C.POINTER(OrcAcOpts), C.c_int32, C.c_int32, _i32p, _f64p, _i32p]
+        L.orc_solve_noise_batch.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, C.c_int32, C.POINTER(_f64p),
+                                            _f64p, C.POINTER(OrcNoiseOpts), C.c_int32, _f64p, _f64p, _i32p]
         L.orc_arbiter_analyze.argtypes = [C.POINTER(_abi.NrNetlist), _f64p, _f64p, _f64p, C.c_double, _f64p, _f64p,
                                           _f64p, _f64p, C.POINTER(C.c_int32), _f64p]
         L.orc_wave_capacity.restype = C.c_int32
@@ -282,6 +292,36 @@ def solve_ac_batch(nl, topo: Topology, fields: dict, B: int, v_op, f_start=1.0, 
     if rc != 0:
         raise ValueError(f"orc_solve_ac_batch: invalid arguments (status {rc})")
     return freqs, out[..., 0] + 1j * out[..., 1], status
+
+
+def solve_noise_batch(nl, topo: Topology, fields: dict, B: int, v_op, f_start=1.0, f_stop=1e9, n_points=50,
+                      output_node=1, temp_K=300.15, mode=NOISE_PER_SOURCE, threads=0):
+    """Noise analysis of B instances at their operating points v_op [B, n]:
+    (freqs [F], output noise PSD [B, F] in V^2/Hz, integrated RMS noise [B],
+    status [B]).  mode NOISE_PER_SOURCE is the reference's one solve per noise
+    source, NOISE_ADJOINT one transposed solve per frequency."""
+    c, keep = nl.to_c()
+    arrs = (_f64p * _abi.N_FIELDS)()
+    hold = []
+    for fid in range(_abi.N_FIELDS):
+        if fid in fields:
+            a = np.ascontiguousarray(fields[fid][:B], dtype=np.float64)
+            hold.append(a)
+            arrs[fid] = _p64(a)
+        else:
+            arrs[fid] = None
+    v = np.ascontiguousarray(v_op, dtype=np.float64).reshape(B, nl.num_nodes)
+    o = OrcNoiseOpts(f_start, f_stop, n_points, output_node, temp_K, mode, 0)
+    freqs = np.zeros(n_points)
+    lib().orc_ac_freqs(C.byref(OrcAcOpts(f_start, f_stop, n_points, 0, 1.0)), _p64(freqs))
+    psd = np.zeros((B, n_points))
+    vrms = np.zeros(B)
+    status = np.zeros(B, dtype=np.int32)
+    rc = lib().orc_solve_noise_batch(C.byref(c), topo.h, B, arrs, _p64(v), C.byref(o), threads, _p64(psd),
+                                     _p64(vrms), _p32(status))
+    if rc != 0:
+        raise ValueError(f"orc_solve_noise_batch: invalid arguments (status {rc})")
+    return freqs, psd, vrms, status
 
 
 def eval_diodes(a, c, Is, N, Vt, v):

@@ -184,3 +184,16 @@ def solve_ac(nl, v_op, f_start, f_stop, n_points, input_node=0, amplitude=1.0):
     if rc:
         raise RuntimeError("ref_ac failed")
     return freqs, out[..., 0] + 1j * out[..., 1]
+
+
+def solve_noise(nl, v_op, output_node, f_start, f_stop, n_points, temp_K=300.15):
+    """ACSolver::solveNoise on the reference's own code: (freqs [F], PSD [F],
+    integrated RMS noise)."""
+    c, keep = nl.to_c()
+    v = np.ascontiguousarray(v_op, dtype=np.float64)
+    freqs, psd, vrms = np.zeros(n_points), np.zeros(n_points), C.c_double(0.0)
+    rc = lib().ref_noise(C.byref(c), _p64(v), output_node, C.c_double(f_start), C.c_double(f_stop), n_points,
+                         C.c_double(temp_K), _p64(freqs), _p64(psd), C.byref(vrms))
+    if rc:
+        raise RuntimeError("ref_noise failed")
+    return freqs, psd, vrms.value

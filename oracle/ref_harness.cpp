@@ -270,4 +270,18 @@ int ref_ac(const nr_netlist* nl, const double* v_op, double f_start, double f_st
     return 0;
 }
 
+int ref_noise(const nr_netlist* nl, const double* v_op, int32_t output_node, double f_start, double f_stop,
+              int32_t n_points, double temp_K, double* freqs, double* psd, double* vrms) {
+    TensorNetlist t = make_netlist(nl);
+    std::vector<double> dc(v_op, v_op + nl->num_nodes);
+    ACSolver::NoiseResult r = ACSolver::solveNoise(t, dc, output_node, 0, f_start, f_stop, n_points, temp_K);
+    if ((int)r.frequencies.size() != n_points) return 1;
+    for (int f = 0; f < n_points; ++f) {
+        freqs[f] = r.frequencies[f];
+        psd[f] = r.output_noise_psd[f];
+    }
+    *vrms = r.integrated_noise_vrms;
+    return 0;
+}
+
 }  // extern "C"

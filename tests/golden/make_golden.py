@@ -97,7 +97,7 @@ def topology_fixture(nl: Netlist) -> dict:
     out["stamp_rows"], out["stamp_cols"] = rows, cols
     rp, ci, slot = R.pattern(nl.num_nodes, rows, cols)
     out["ref_row_ptr"], out["ref_col_idx"], out["ref_stamp_slot"] = rp, ci, slot
-    # D3/D4 projection of the reference ordering onto rows
+    # projection of the reference ordering onto rows
     seen, perm0 = set(), []
     for i in ordering:
         if 1 <= i <= nl.num_nodes and i not in seen:

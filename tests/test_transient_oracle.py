@@ -56,7 +56,7 @@ def test_pulse_waveform(oracle_lib):
 
 
 def test_inverter_chain_transient_switches(oracle_lib):
-    """inv16, zero start (D11), 200 steps of 1 ns: every step converges,
+    """inv16, zero start, 200 steps of 1 ns: every step converges,
     the chain settles before the input edge at 20 ns and the edge reaches
     the last stage afterwards."""
     O = oracle_lib

@@ -24,7 +24,7 @@ HBM.  Prints ONE JSON line:
 
 Other workloads (--workload): logic (random CMOS logic, DC and transient),
 mixed (uneven circuit sizes), ac (AC sweeps of the mesh with capacitors),
-noise (noise analysis of the same circuit).
+noise (noise analysis of the same circuit), hightw (high-treewidth logic).
 """
 from __future__ import annotations
 
@@ -149,6 +149,24 @@ def roofline(kst, traffic=True):
             "algorithmic_bytes_per_launch": d["bytes"] / d["launches"],
             "mean_launch_us": d["ms"] * 1e3 / d["launches"], "launches": d["launches"],
             "device_ms_share": d["ms"] / sum(x["ms"] for x in kst.values())}
+
+
+FP64_PEAK_TFS = 78.6           # MI355X FP64, public spec
+
+
+def roofline_either(kst):
+    """The dominant kernel against whichever of HBM bandwidth and the FP64
+    peak it is closer to (its algorithmic bytes and flops per launch)."""
+    dom = max(kst, key=lambda n: kst[n]["ms"])
+    d = kst[dom]
+    gbs = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    tfs = d["flops"] / (d["ms"] * 1e-3) / 1e12
+    mem = gbs / HBM_PEAK_GBS >= tfs / FP64_PEAK_TFS
+    ach, peak = (gbs, HBM_PEAK_GBS) if mem else (tfs, FP64_PEAK_TFS)
+    return {"bound": "hbm" if mem else "fp64", "achieved": ach, "peak": peak, "unit": "GB/s" if mem else "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "kernel": dom, "algorithmic_bytes_per_launch": d["bytes"] / d["launches"],
+            "algorithmic_flops_per_launch": d["flops"] / d["launches"], "mean_launch_us": d["ms"] * 1e3 / d["launches"],
+            "launches": d["launches"], "device_ms_share": d["ms"] / sum(x["ms"] for x in kst.values())}
 
 
 def kernel_table(kst):
@@ -464,6 +482,60 @@ def noise_workload(a):
     return 0
 
 
+def hightw_workload(a):
+    """DC operating points of high-treewidth random logic: NAND2 / INV gates
+    whose inputs come from 65 slots of the previous six levels (no locality),
+    a.hightw_nodes nodes (2000: treewidth 214, fronts up to order 215 with
+    their panels in HBM), Monte-Carlo instances with VTO sigma 30 mV and KP
+    sigma 3 %.  The reference sends circuits of treewidth above 50 away from
+    its sparse LU (to tensor contraction, Schur decomposition or PCG with a
+    dense fallback); here they run on the same multifrontal factorisation.
+    Launch shapes autotuned first (untimed)."""
+    from karam_amd import _abi, netlists, nr
+    B = a.instances if a.instances != 4096 else 1024
+    nl = netlists.random_logic(a.hightw_nodes, seed=3, spread=32, window=6, width=64)
+    f = netlists.mc_fields(nl, B, seed=1000, sigma_vto=0.03, sigma_kp=0.03)
+    ctx = nr.Context(0)
+    t = time.perf_counter()
+    T = nr.Topology(nl, ctx)
+    symbolic_ms = (time.perf_counter() - t) * 1e3
+    info = {k: T.info[k] for k in ("n", "nnz_f", "treewidth", "max_front", "n_supernodes")}
+    log(f"hightw {info} symbolic {symbolic_ms:.0f} ms")
+    bt = nr.Batch(ctx, T, B)
+    bt.set_fields(f)
+    tune = None
+    if not a.no_tune:
+        tu = bt.autotune(precision=_abi.NR_PREC_FP64, reps=2)
+        tune = {"factor_solve_ms_before": tu["ms_before"], "factor_solve_ms_after": tu["ms_after"]}
+        log(f"autotune: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution")
+    for _ in range(a.warmup):
+        bt.solve_dc()
+    rate, ms, kst, st, r = timed_solves(bt, a.steps)
+    V = bt.voltages()
+    it = st["iterations"]
+    rate32, ms32, kst32, st32, _ = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE)
+    out = {
+        "metric": "MC circuit solves/sec at KCL resid <=1e-9 A (high-treewidth logic)", "value": rate,
+        "unit": "solves/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic: seeded random NAND2/INV logic without locality, per-instance VTO / KP",
+        "config": {"workload": f"random logic, {info['n']} nodes, treewidth {info['treewidth']}", "instances": B,
+                   **info, "symbolic_ms": symbolic_ms, "autotune": tune},
+        "iterations_mean": float(it.mean()), "converged": int((st["status"] == 0).sum()),
+        "roofline": roofline_either(kst), "kernels": kernel_table(kst),
+        "fp32_refine": {"value": rate32, "ms_per_step": ms32, "roofline": roofline_either(kst32),
+                        "max_err_vs_fp64": float(np.max(np.abs(bt.voltages() - V)))},
+        "cpu_baseline": None,
+    }
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        base, parity = cpu_baseline(nl, f, B, V, it, a.cpu_seconds, threads)
+        out["cpu_baseline"], out["parity_sample"] = base, parity
+        log(f"hightw cpu baseline {base['value']:.1f} solves/s; parity {parity}")
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def mixed_workload(a):
     """Uneven sizes: a.mixed_instances instances whose node counts are
     log-uniform in [200, 50000], the family of instance i being i mod 3 (the
@@ -563,11 +635,12 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
-    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac", "noise"], default="rdmesh64")
+    ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac", "noise", "hightw"], default="rdmesh64")
     ap.add_argument("--ac-points", type=int, default=50)
     ap.add_argument("--mixed-instances", type=int, default=10000)
     ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--logic-nodes", type=int, default=2000)
+    ap.add_argument("--hightw-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
     ap.add_argument("--copy-probe", action="store_true")
@@ -588,6 +661,8 @@ def main():
         return ac_workload(a)
     if a.workload == "noise":
         return noise_workload(a)
+    if a.workload == "hightw":
+        return hightw_workload(a)
 
     from karam_amd import _abi, netlists, nr
 

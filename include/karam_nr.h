@@ -265,6 +265,8 @@ typedef struct nr_solve_opts {
     int32_t fallback_plain;  /* 1: an instance whose seeded solve does not
                                 converge is solved again from the plain
                                 GMIN-stepping start                           */
+    int32_t trace_capacity;  /* > 0: keep each instance's last trace_capacity
+                                ConvergenceStep records (nr_batch_get_trace) */
 } nr_solve_opts;
 NR_API void nr_solve_opts_default(nr_solve_opts* opts);
 
@@ -487,6 +489,46 @@ typedef struct nr_instance_stats {
 } nr_instance_stats;
 NR_API int nr_batch_get_stats(const nr_batch* batch, nr_instance_stats* out,
                               int64_t count);
+
+/* NR convergence diagnostics of the last DC solve: SolverStats' fields
+ * (infrastructure/simtrace.h:76-128) as solveDC fills them
+ * (solvers/circuitsim_solver.cpp:407-444, 489-551), per instance.  The
+ * failure class is ConvergenceEngine::classifyFailure of the instance's whole
+ * history (solvers/convergence_engine.h:26-87).  Not provided: the LU
+ * residual and smallest pivot of the reference's pivoting LU (the batched LU
+ * does not pivot), the worst-device list and the text fields. */
+enum {
+    NR_FAIL_NONE = 0, NR_FAIL_SINGULAR = 1, NR_FAIL_DIVERGENCE = 2, NR_FAIL_LIMIT_CYCLE = 3,
+    NR_FAIL_SLOW = 4, NR_FAIL_FLOATING_NODE = 5, NR_FAIL_OVERFLOW = 6
+};
+typedef struct nr_diagnostics {
+    int64_t history;             /* ConvergenceStep records (Newton iterations
+                                    that reached the update)                 */
+    int32_t failure;             /* NR_FAIL_* (ConvergenceFailureType)        */
+    int32_t stagnation;          /* stagnationDetected                       */
+    int32_t divergence;          /* divergenceDetected                       */
+    int32_t worst_node;          /* worstNodeIndex: 1-based, largest |KCL| at
+                                    the result (0 if every residual is 0)    */
+    double worst_node_v;         /* worstNodeVoltage                          */
+    double rms_residual;         /* rmsResidual (A)                          */
+    int32_t gmin_stepping_used;  /* gminSteppingUsed: the last GMIN value is
+                                    below 1e-3                               */
+    int32_t pad;
+} nr_diagnostics;
+/* One ConvergenceStep (simtrace.h:30-37): residualNorm (max |KCL| after the
+ * update), maxVoltageDelta, dampingFactor (0.5 when the damped residual is
+ * still not below the last one), worstNode as solveDC fills it (node 1), with
+ * the GMIN step and the iteration within it. */
+typedef struct nr_conv_step {
+    double residual, max_dv, damping;
+    int32_t gmin_step, iteration, worst_node, pad;
+} nr_conv_step;
+NR_API int nr_batch_get_diagnostics(const nr_batch* batch, nr_diagnostics* out, int64_t count);
+/* The ring of the last solve with trace_capacity = cap > 0: out[b][k], count
+ * = B * cap; record j of instance b (0-based, in the order recorded) is at k
+ * = j mod cap, and nr_diagnostics.history gives how many were recorded;
+ * slots not reached hold zeros. */
+NR_API int nr_batch_get_trace(const nr_batch* batch, nr_conv_step* out, int64_t count);
 
 /* Per-phase device time of the last solve (milliseconds, summed over
  * launches) when record_timing was set. */

@@ -108,8 +108,27 @@ class NrSolveOpts(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("precision", C.c_int32),
         ("seeded_start_step", C.c_int32), ("record_timing", C.c_int32), ("use_graph", C.c_int32),
-        ("fallback_plain", C.c_int32),
+        ("fallback_plain", C.c_int32), ("trace_capacity", C.c_int32),
     ]
+
+
+class NrDiagnostics(C.Structure):
+    _fields_ = [
+        ("history", C.c_int64), ("failure", C.c_int32), ("stagnation", C.c_int32), ("divergence", C.c_int32),
+        ("worst_node", C.c_int32), ("worst_node_v", C.c_double), ("rms_residual", C.c_double),
+        ("gmin_stepping_used", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+class NrConvStep(C.Structure):
+    _fields_ = [
+        ("residual", C.c_double), ("max_dv", C.c_double), ("damping", C.c_double),
+        ("gmin_step", C.c_int32), ("iteration", C.c_int32), ("worst_node", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+NR_FAIL_NONE, NR_FAIL_SINGULAR, NR_FAIL_DIVERGENCE, NR_FAIL_LIMIT_CYCLE, NR_FAIL_SLOW, NR_FAIL_FLOATING_NODE, \
+    NR_FAIL_OVERFLOW = range(7)
 
 
 class NrTransientOpts(C.Structure):

@@ -142,6 +142,28 @@ struct TranState {
     long long rec;               // waveform entries written
     int method, attempt, instab, accepted, rejected, switched_at, glin_method, dec;
 };
+// NR convergence diagnostics of a DC solve (solveDC's ConvergenceStep records
+// and SolverStats fields, solvers/circuitsim_solver.cpp:407-444, 489-551).
+// classifyFailure (solvers/convergence_engine.h:26-87) reads the whole
+// history only through whether some max |dv| exceeded 100 V, the last ten
+// residuals, the number of residual direction changes and the length, which
+// DiagState keeps record by record; ConvRec is one record (nr_conv_step) of
+// the optional per-instance ring; FinalDiag the per-node KCL statistics at
+// the result.
+struct DiagState {
+    double last[10];             // residual of record k at k % 10
+    long long n;                 // records
+    int changes, big_dv, stagnation, divergence;
+};
+struct ConvRec {
+    double residual, max_dv, damping;
+    int gmin_step, iteration, worst_node, pad;
+};
+struct FinalDiag {
+    double rms, worst_v;
+    int worst_node, pad;
+};
+
 // InstState.done in a transient: 0 solving, 1 finished, 2 the attempt's
 // Newton solve converged, 3 it failed (the step control decides), 4 the
 // next attempt is armed (k_tran_begin sets it up)
@@ -180,6 +202,12 @@ struct DevBatch {
     // path does not otherwise use
     double* res;
     int p32;
+    // DC diagnostics: per-instance state, the optional ring of the last
+    // trace_cap records per instance, the final per-node statistics
+    DiagState* diag;
+    ConvRec* trace;
+    int trace_cap;
+    FinalDiag* fin;
     InstState* st;
     int* act;
     int* nact;
@@ -697,6 +725,12 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         s.seeded = use_seed;
         s.prev_kcl = 0.0; s.kcl = 0.0;
         P.st[b] = s;
+        if (mode == 0) {
+            DiagState d;
+            for (int k = 0; k < 10; ++k) d.last[k] = 0.0;
+            d.n = 0; d.changes = 0; d.big_dv = 0; d.stagnation = 0; d.divergence = 0;
+            P.diag[b] = d;
+        }
         P.act[b] = b;
         if (b == 0) *P.nact = P.B;
     }
@@ -1789,6 +1823,35 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
 // attempt leaves the active list and k_tran_control decides.  On the FP32 path the solve returned the correction,
 // so the new iterate is v + x (FP64), and every KCL audit also stores its
 // per-node vector (the next Newton rhs).
+// one ConvergenceStep of instance b (solvers/circuitsim_solver.cpp:407-441):
+// the damping factor reads 0.5 when the (damped) residual is still not below
+// the last one; the worst node as the reference fills it, from a vector that
+// holds the residual at every node (so node 1); stagnation when the step's
+// iteration is past the first and the residual moved by less than 1e-15
+// above KCL_TOL (1e-12, physics/physics_constants.h:76); divergence when the
+// last three residuals rise
+__device__ void diag_record(const DevBatch& P, int b, const InstState& st, double kcl, double dv, double prev) {
+    DiagState d = P.diag[b];
+    const double damping = (st.it > 0 && kcl >= prev) ? 0.5 : 1.0;
+    if (P.trace_cap > 0) {
+        ConvRec r;
+        r.residual = kcl; r.max_dv = dv; r.damping = damping;
+        r.gmin_step = st.step; r.iteration = st.it; r.worst_node = 1; r.pad = 0;
+        P.trace[(size_t)b * P.trace_cap + (size_t)(d.n % P.trace_cap)] = r;
+    }
+    if (st.it > 0 && fabs(prev - kcl) < 1e-15 && kcl > 1e-12) d.stagnation = 1;
+    if (dv > 100.0) d.big_dv = 1;
+    d.last[d.n % 10] = kcl;
+    d.n++;
+    if (d.n >= 3) {
+        const double h1 = d.last[(d.n - 1) % 10], h2 = d.last[(d.n - 2) % 10], h3 = d.last[(d.n - 3) % 10];
+        if (h1 > h2 && h2 > h3) d.divergence = 1;
+        const double pd = h2 - h3, cd = h1 - h2;
+        if ((pd > 0 && cd < 0) || (pd < 0 && cd > 0)) d.changes++;
+    }
+    P.diag[b] = d;
+}
+
 __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
                                                 int nsteps, int tran) {
 #pragma clang fp contract(off)
@@ -1843,6 +1906,7 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         __syncthreads();
     }
     double kcl = block_kcl(T, P, b, vn, red, res);
+    const double prev_final = st.prev_kcl;
     if (!tran && st.it > 0 && kcl >= st.prev_kcl) {
         for (int pass = 0; pass < 2; ++pass) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
@@ -1865,7 +1929,16 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         ok = block_and(ok, redi);
         conv = ok && !(kcl > tol);
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v[i] = vn[i];
+    double dv = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double d = fabs(vn[i] - v[i]);
+        if (d > dv) dv = d;
+        v[i] = vn[i];
+    }
+    if (!tran) {
+        dv = block_max(dv, red);
+        if (threadIdx.x == 0) diag_record(P, b, st, kcl, dv, prev_final);
+    }
     if (threadIdx.x == 0 && tran) {
         st.kcl = kcl;                         // the last iterate's audit
         if (conv) {
@@ -2120,11 +2193,52 @@ __global__ void k_noise_integrate(const double* __restrict__ freqs, const double
     vrms[b] = sqrt(integrated > 0.0 ? integrated : 0.0);
 }
 
+// The final audit of each instance: the max |KCL| and, from the per-node
+// residuals (staged in vn, free after the solve), their RMS (squares added
+// in node order), the first node with the largest |residual| and its voltage
+// (solvers/circuitsim_solver.cpp:489-508).
 __global__ __launch_bounds__(256) void k_final_kcl(DevTopo T, DevBatch P) {
+#pragma clang fp contract(off)
     __shared__ double red[8];
-    const int b = blockIdx.x;
-    double k = block_kcl(T, P, b, P.v + (size_t)b * T.n, red);
-    if (threadIdx.x == 0) P.st[b].kcl = k;
+    __shared__ double wbest[8];
+    __shared__ int widx[8];
+    const int b = blockIdx.x, n = T.n;
+    const double* v = P.v + (size_t)b * n;
+    double* rr = P.vn + (size_t)b * n;
+    double k = block_kcl(T, P, b, v, red, rr);
+    __syncthreads();
+    const double ss = block_ordered_sum(n, [&](int i) {
+        const double a = fabs(rr[i]);
+        return a * a;
+    });
+    double best = 0.0;
+    int bi = -1;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        const double a = fabs(rr[i]);
+        if (a > best) { best = a; bi = i; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_down(best, o);
+        const int oi = __shfl_down(bi, o);
+        if (oi >= 0 && (ob > best || (ob == best && (bi < 0 || oi < bi)))) { best = ob; bi = oi; }
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { wbest[w] = best; widx[w] = bi; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) {
+            const double ob = wbest[q];
+            const int oi = widx[q];
+            if (oi >= 0 && (ob > best || (ob == best && (bi < 0 || oi < bi)))) { best = ob; bi = oi; }
+        }
+        P.st[b].kcl = k;
+        FinalDiag f;
+        f.rms = sqrt(ss / (double)n);
+        f.worst_node = bi + 1;
+        f.worst_v = bi >= 0 ? v[bi] : 0.0;
+        f.pad = 0;
+        P.fin[b] = f;
+    }
 }
 
 // standalone evaluation slots

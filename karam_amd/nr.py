@@ -47,6 +47,7 @@ EXPORTED = [
     "nr_spice_parse", "nr_spice_free", "nr_spice_get_netlist", "nr_spice_get_info", "nr_spice_node_name",
     "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
     "nr_ac_opts_default", "nr_batch_ac", "nr_batch_get_ac", "nr_batch_get_ac_freqs", "nr_batch_get_ac_status",
+    "nr_batch_get_diagnostics", "nr_batch_get_trace",
     "nr_noise_opts_default", "nr_batch_noise", "nr_batch_get_noise", "nr_batch_get_noise_vrms",
     "nr_batch_get_noise_freqs", "nr_batch_get_noise_status",
 ]
@@ -81,6 +82,8 @@ def lib():
         L.nr_batch_get_ac.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_ac_freqs.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_ac_status.argtypes = [_vp, C.POINTER(C.c_int32), C.c_int64]
+        L.nr_batch_get_diagnostics.argtypes = [_vp, C.POINTER(_abi.NrDiagnostics), C.c_int64]
+        L.nr_batch_get_trace.argtypes = [_vp, C.POINTER(_abi.NrConvStep), C.c_int64]
         L.nr_noise_opts_default.argtypes = [C.POINTER(_abi.NrNoiseOpts)]
         L.nr_noise_opts_default.restype = None
         L.nr_batch_noise.argtypes = [_vp, C.POINTER(_abi.NrNoiseOpts), C.POINTER(_abi.NrResult)]
@@ -279,12 +282,12 @@ class Topology:
 
 
 def solve_opts(max_iter=50, gmin_steps=19, precision=_abi.NR_PREC_FP64, seeded_start_step=18,
-               record_timing=0, use_graph=0, fallback_plain=1):
+               record_timing=0, use_graph=0, fallback_plain=1, trace_capacity=0):
     o = _abi.NrSolveOpts()
     lib().nr_solve_opts_default(C.byref(o))
     o.max_iter, o.gmin_steps, o.precision = max_iter, gmin_steps, precision
     o.seeded_start_step, o.record_timing, o.use_graph = seeded_start_step, record_timing, use_graph
-    o.fallback_plain = fallback_plain
+    o.fallback_plain, o.trace_capacity = fallback_plain, trace_capacity
     return o
 
 
@@ -321,7 +324,25 @@ class Batch:
         o = opts if opts is not None else solve_opts(**kw)
         r = _abi.NrResult()
         _check(lib().nr_batch_solve_dc(self.h, C.byref(o), C.byref(r)), self.ctx)
+        self._trace_cap = int(o.trace_capacity)
         return {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
+
+    def diagnostics(self):
+        """SolverStats diagnostics of the last DC solve, per instance:
+        {history, failure (NR_FAIL_*), stagnation, divergence, worst_node,
+        worst_node_v, rms_residual, gmin_stepping_used} arrays."""
+        arr = (_abi.NrDiagnostics * self.B)()
+        _check(lib().nr_batch_get_diagnostics(self.h, arr, self.B), self.ctx)
+        return {k: np.array([getattr(x, k) for x in arr]) for k, _ in _abi.NrDiagnostics._fields_ if k != "pad"}
+
+    def trace(self):
+        """The last solve's ConvergenceStep rings: {field: [B, capacity]}
+        (record j of an instance at j mod capacity)."""
+        cap = getattr(self, "_trace_cap", 0)
+        arr = (_abi.NrConvStep * (self.B * cap))()
+        _check(lib().nr_batch_get_trace(self.h, arr, self.B * cap), self.ctx)
+        return {k: np.array([getattr(x, k) for x in arr]).reshape(self.B, cap)
+                for k, _ in _abi.NrConvStep._fields_ if k != "pad"}
 
     def transient(self, h=1e-9, steps=200, max_iter=50, start=_abi.NR_TRAN_START_ZERO, probes=None,
                   record_timing=0, precision=_abi.NR_PREC_FP64, use_graph=0, method=_abi.NR_INT_GEAR1,

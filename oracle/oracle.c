@@ -1416,6 +1416,61 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 void orc_dc_opts_default(orc_dc_opts* o) {
     o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_MULTIFRONTAL;
     o->start_step = 0; o->use_seed = 0; o->precision = ORC_PREC_FP64;
+    o->trace_cap = 0; o->pad = 0; o->trace = NULL;
+}
+
+/* The NR convergence diagnostics of solveDC (solvers/circuitsim_solver.cpp:
+ * 407-444, 489-551) kept incrementally: classifyFailure (solvers/
+ * convergence_engine.h:26-87) reads the whole history, but only through
+ * whether some max |dv| exceeds 100 V, the last ten residuals, the number of
+ * residual direction changes and the history length, which are updated
+ * record by record here. */
+typedef struct {
+    double last[10];       /* residual of record k at k % 10 */
+    int64_t n;
+    int32_t changes, big_dv, stagnation, divergence;
+} dc_diag;
+
+static void diag_record(dc_diag* d, double residual, double max_dv, double prev_final, int32_t it) {
+    /* stagnation (:430-434): the step's iteration > 0, the residual within
+     * 1e-15 of the last and above KCL_TOL (1e-12, physics_constants.h:76) */
+    if (it > 0 && fabs(prev_final - residual) < 1e-15 && residual > 1e-12) d->stagnation = 1;
+    if (max_dv > 100.0) d->big_dv = 1;
+    d->last[d->n % 10] = residual;
+    d->n++;
+    if (d->n >= 3) {
+        const double h1 = d->last[(d->n - 1) % 10], h2 = d->last[(d->n - 2) % 10], h3 = d->last[(d->n - 3) % 10];
+        if (h1 > h2 && h2 > h3) d->divergence = 1;           /* :436-441 */
+        const double pd = h2 - h3, cd = h1 - h2;                /* classifyFailure :63-69 */
+        if ((pd > 0 && cd < 0) || (pd < 0 && cd > 0)) d->changes++;
+    }
+}
+
+static int32_t diag_classify(const dc_diag* d, int singular) {
+    if (singular) return ORC_FAIL_SINGULAR;
+    if (d->n == 0) return ORC_FAIL_NONE;
+    if (d->big_dv) return ORC_FAIL_OVERFLOW;
+    if (d->n >= 5) {
+        int up = 1;
+        for (int64_t i = d->n - 4; i < d->n; ++i)
+            if (d->last[i % 10] <= d->last[(i - 1) % 10]) { up = 0; break; }
+        if (up) return ORC_FAIL_DIVERGENCE;
+    }
+    if (d->n >= 4 && (double)d->changes / (double)(d->n - 2) > 0.6) return ORC_FAIL_LIMIT_CYCLE;
+    if (d->n >= 10) {
+        const double initial = d->last[(d->n - 10) % 10], final_val = d->last[(d->n - 1) % 10];
+        if (initial > 0.0 && final_val > 0.5 * initial) return ORC_FAIL_SLOW;
+    }
+    return ORC_FAIL_NONE;
+}
+
+/* the incremental classification fed one record at a time (checked against
+ * the reference's classifyFailure on whole histories) */
+int32_t orc_classify_history(const double* residual, const double* max_dv, int64_t n, int32_t singular) {
+    dc_diag d;
+    memset(&d, 0, sizeof(d));
+    for (int64_t k = 0; k < n; ++k) diag_record(&d, residual[k], max_dv[k], k > 0 ? residual[k - 1] : 0.0, 0);
+    return diag_classify(&d, singular);
 }
 
 int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opts,
@@ -1442,10 +1497,13 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     else orc_seed(nl, v);
     int converged = 0, singular = 0;
     int32_t total = 0, ref_total = 0, steps_done = 0;
-    double final_kcl = 0.0;
+    double final_kcl = 0.0, last_gmin = 1.0;
+    dc_diag dg;
+    memset(&dg, 0, sizeof(dg));
     for (int32_t k = o.start_step; k < o.gmin_steps; ++k) {
         double gmin = pow(10.0, -(3.0 + k * 0.5));
         if (gmin < 1e-12) gmin = 1e-12;
+        last_gmin = gmin;
         int gconv = 0, sing_step = 0;
         int32_t it = 0;
         for (; it < o.max_iter; ++it) {
@@ -1477,6 +1535,24 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
                     kcl = kcl_soa(nl, s, vn, res, NULL);
                 }
             }
+            {
+                /* ConvergenceStep (:407-425): damping 0.5 when the (damped)
+                 * residual is still not below the last; worst node as the
+                 * reference fills it, from a vector holding the residual at
+                 * every node (so node 1) */
+                double max_dv = 0.0;
+                for (int32_t i = 0; i < n; ++i) {
+                    const double d = fabs(vn[i] - v[i]);
+                    if (d > max_dv) max_dv = d;
+                }
+                const double damping = (it > 0 && kcl >= final_kcl) ? 0.5 : 1.0;
+                if (o.trace_cap > 0 && o.trace) {
+                    orc_conv_step* r = &o.trace[dg.n % o.trace_cap];
+                    r->residual = kcl; r->max_dv = max_dv; r->damping = damping;
+                    r->gmin_step = k; r->iteration = it; r->worst_node = 1; r->pad = 0;
+                }
+                diag_record(&dg, kcl, max_dv, final_kcl, it);
+            }
             final_kcl = kcl;
             double tol = (k == o.gmin_steps - 1) ? 1e-12 : 1.0;
             if (orc_check_convergence(n, vn, v, kcl, tol)) {
@@ -1495,6 +1571,26 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         if (!gconv && !(final_kcl < 1.0)) break;
     }
     double fk = kcl_soa(nl, s, v, res, NULL);
+    {
+        /* final per-node residuals: RMS, worst node and its voltage
+         * (:489-508) */
+        double sum_sq = 0.0, max_abs = 0.0;
+        int32_t worst = 0;
+        double worst_v = 0.0;
+        for (int32_t i = 0; i < n; ++i) {
+            const double a = fabs(res[i]);
+            sum_sq += a * a;
+            if (a > max_abs) { max_abs = a; worst = i + 1; worst_v = v[i]; }
+        }
+        out->rms = sqrt(sum_sq / (double)n);
+        out->worst_node = worst;
+        out->worst_v = worst_v;
+    }
+    out->history = dg.n;
+    out->failure = diag_classify(&dg, singular);
+    out->stagnation = dg.stagnation;
+    out->divergence = dg.divergence;
+    out->gmin_used = last_gmin < 1e-3;
     out->iterations = total;
     out->ref_iterations = ref_total;
     out->status = converged ? NR_STATUS_CONVERGED : (singular ? NR_STATUS_SINGULAR : NR_STATUS_NOT_CONVERGED);
@@ -1527,7 +1623,10 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
         }
-        orc_solve_dc(&inst, t, opts, v_out + (size_t)b * n, &outs[b]);
+        orc_dc_opts o;
+        if (opts) o = *opts; else orc_dc_opts_default(&o);
+        if (o.trace) o.trace += (size_t)b * (size_t)o.trace_cap;
+        orc_solve_dc(&inst, t, &o, v_out + (size_t)b * n, &outs[b]);
     }
     return NR_OK;
 }

@@ -94,6 +94,13 @@ enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2 };
  * computed in FP64 and updates v in FP64; limiting, KCL and the convergence
  * test stay FP64. */
 enum { ORC_PREC_FP64 = 0, ORC_PREC_FP32_REFINE = 1 };
+/* one ConvergenceStep record (infrastructure/simtrace.h:30-37) */
+typedef struct orc_conv_step {
+    double residual;      /* max |KCL| after the update (A) */
+    double max_dv;        /* max |v_new - v_old| (V)        */
+    double damping;       /* 0.5 or 1.0, as the reference sets it */
+    int32_t gmin_step, iteration, worst_node, pad;
+} orc_conv_step;
 typedef struct orc_dc_opts {
     int32_t max_iter;     /* 50 */
     int32_t gmin_steps;   /* 19 */
@@ -101,6 +108,9 @@ typedef struct orc_dc_opts {
     int32_t start_step;   /* GMIN step to start at (seeded start) */
     int32_t use_seed;     /* 1: v_inout holds the start vector */
     int32_t precision;    /* ORC_PREC_* */
+    int32_t trace_cap;    /* > 0: ring of the last trace_cap records */
+    int32_t pad;
+    orc_conv_step* trace; /* [trace_cap] (per instance in the batch call) */
 } orc_dc_opts;
 typedef struct orc_dc_out {
     int32_t iterations;      /* true NR iteration count */
@@ -109,8 +119,21 @@ typedef struct orc_dc_out {
     int32_t gmin_steps_done;
     double kcl;
     int32_t iters_per_step[32];
+    /* SolverStats diagnostics (infrastructure/simtrace.h:76-128) */
+    int64_t history;         /* ConvergenceStep records of the solve */
+    int32_t failure;         /* ORC_FAIL_* (classifyFailure)          */
+    int32_t stagnation, divergence;
+    int32_t worst_node;      /* 1-based, largest |KCL| at the result  */
+    int32_t gmin_used;       /* the last GMIN value is below 1e-3     */
+    int32_t pad;
+    double rms;              /* RMS of the per-node KCL at the result */
+    double worst_v;          /* the worst node's voltage              */
 } orc_dc_out;
+/* ConvergenceFailureType (infrastructure/simtrace.h:39-47) */
+enum { ORC_FAIL_NONE = 0, ORC_FAIL_SINGULAR = 1, ORC_FAIL_DIVERGENCE = 2, ORC_FAIL_LIMIT_CYCLE = 3,
+       ORC_FAIL_SLOW = 4, ORC_FAIL_FLOATING = 5, ORC_FAIL_OVERFLOW = 6 };
 void orc_dc_opts_default(orc_dc_opts* o);
+int32_t orc_classify_history(const double* residual, const double* max_dv, int64_t n, int32_t singular);
 int orc_solve_dc(const nr_netlist* nl, const orc_topo* t,
                  const orc_dc_opts* opts, double* v_inout, orc_dc_out* out);
 

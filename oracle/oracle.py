@@ -20,9 +20,15 @@ _i32p = C.POINTER(C.c_int32)
 _f64p = C.POINTER(C.c_double)
 
 
+class OrcConvStep(C.Structure):
+    _fields_ = [("residual", C.c_double), ("max_dv", C.c_double), ("damping", C.c_double),
+                ("gmin_step", C.c_int32), ("iteration", C.c_int32), ("worst_node", C.c_int32), ("pad", C.c_int32)]
+
+
 class OrcDcOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("lu_kind", C.c_int32),
-                ("start_step", C.c_int32), ("use_seed", C.c_int32), ("precision", C.c_int32)]
+                ("start_step", C.c_int32), ("use_seed", C.c_int32), ("precision", C.c_int32),
+                ("trace_cap", C.c_int32), ("pad", C.c_int32), ("trace", C.POINTER(OrcConvStep))]
 
 
 class OrcTranOpts(C.Structure):
@@ -54,7 +60,13 @@ NOISE_PER_SOURCE, NOISE_ADJOINT = 0, 1
 class OrcDcOut(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
                 ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
-                ("iters_per_step", C.c_int32 * 32)]
+                ("iters_per_step", C.c_int32 * 32),
+                ("history", C.c_int64), ("failure", C.c_int32), ("stagnation", C.c_int32),
+                ("divergence", C.c_int32), ("worst_node", C.c_int32), ("gmin_used", C.c_int32),
+                ("pad", C.c_int32), ("rms", C.c_double), ("worst_v", C.c_double)]
+
+
+FAIL_NONE, FAIL_SINGULAR, FAIL_DIVERGENCE, FAIL_LIMIT_CYCLE, FAIL_SLOW, FAIL_FLOATING, FAIL_OVERFLOW = range(7)
 
 
 LU_DENSE = 0
@@ -108,6 +120,8 @@ def lib():
         L.orc_vcrit.restype = C.c_double
         L.orc_apply_pn_limits.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, _f64p]
         L.orc_kcl.argtypes = [C.POINTER(_abi.NrNetlist), C.c_void_p, _f64p, _f64p]
+        L.orc_classify_history.argtypes = [_f64p, _f64p, C.c_int64, C.c_int32]
+        L.orc_classify_history.restype = C.c_int32
         L.orc_kcl.restype = C.c_double
         L.orc_check_convergence.argtypes = [C.c_int32, _f64p, _f64p, C.c_double, C.c_double]
         L.orc_eval_diodes.argtypes = [C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, C.c_int32,
@@ -162,8 +176,24 @@ class Topology:
             self.h = None
 
 
-def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0, precision=PREC_FP64):
-    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision)
+def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0, precision=PREC_FP64,
+            trace_cap=0, trace=None):
+    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision, trace_cap, 0, trace)
+
+
+def classify_history(residual, max_dv, singular=False):
+    """The oracle's incremental failure classification of a history."""
+    r = np.ascontiguousarray(residual, np.float64)
+    d = np.ascontiguousarray(max_dv, np.float64)
+    return lib().orc_classify_history(_p64(r), _p64(d), C.c_int64(len(r)), int(singular))
+
+
+def _trace_dict(tr):
+    """ConvergenceStep records as arrays (numpy view of the ctypes array)."""
+    a = np.ctypeslib.as_array(tr).view(np.dtype([("residual", "<f8"), ("max_dv", "<f8"), ("damping", "<f8"),
+                                                 ("gmin_step", "<i4"), ("iteration", "<i4"),
+                                                 ("worst_node", "<i4"), ("pad", "<i4")]))
+    return {k: np.array(a[k]) for k in ("residual", "max_dv", "damping", "gmin_step", "iteration", "worst_node")}
 
 
 def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, gmin_steps=19,
@@ -177,9 +207,12 @@ def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, 
 
 
 def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL, threads=0,
-                   max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0):
+                   max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0, trace_cap=0):
     """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array;
-    seeds (optional, [B, n]) start every instance at GMIN step start_step."""
+    seeds (optional, [B, n]) start every instance at GMIN step start_step.
+    trace_cap > 0 adds stats["trace"]: the last trace_cap ConvergenceStep
+    records of every instance ([B, trace_cap] arrays, record k at k %
+    trace_cap)."""
     c, keep = nl.to_c()
     arrs = (_f64p * _abi.N_FIELDS)()
     hold = []
@@ -192,8 +225,9 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL,
             arrs[fid] = None
     v = np.zeros((B, nl.num_nodes)) if seeds is None else np.array(seeds, dtype=np.float64).reshape(B, -1)
     outs = (OrcDcOut * B)()
+    tr = (OrcConvStep * (B * trace_cap))() if trace_cap > 0 else None
     o = dc_opts(max_iter, gmin_steps, lu, start_step if seeds is not None else 0, 0 if seeds is None else 1,
-                precision)
+                precision, trace_cap, C.cast(tr, C.POINTER(OrcConvStep)) if tr is not None else None)
     lib().orc_solve_dc_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), outs)
     stats = {
         "iterations": np.array([x.iterations for x in outs]),
@@ -201,7 +235,17 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL,
         "status": np.array([x.status for x in outs]),
         "kcl": np.array([x.kcl for x in outs]),
         "gmin_steps_done": np.array([x.gmin_steps_done for x in outs]),
+        "history": np.array([x.history for x in outs]),
+        "failure": np.array([x.failure for x in outs]),
+        "stagnation": np.array([x.stagnation for x in outs]),
+        "divergence": np.array([x.divergence for x in outs]),
+        "worst_node": np.array([x.worst_node for x in outs]),
+        "worst_v": np.array([x.worst_v for x in outs]),
+        "rms": np.array([x.rms for x in outs]),
+        "gmin_used": np.array([x.gmin_used for x in outs]),
     }
+    if tr is not None:
+        stats["trace"] = {k: x.reshape(B, trace_cap) for k, x in _trace_dict(tr).items()}
     return v, stats
 
 

@@ -197,3 +197,10 @@ def solve_noise(nl, v_op, output_node, f_start, f_stop, n_points, temp_K=300.15)
     if rc:
         raise RuntimeError("ref_noise failed")
     return freqs, psd, vrms.value
+
+
+def classify_failure(residual, max_dv, singular=False):
+    """ConvergenceEngine::classifyFailure on the reference's own code."""
+    r = np.ascontiguousarray(residual, np.float64)
+    d = np.ascontiguousarray(max_dv, np.float64)
+    return lib().ref_classify_failure(_p64(r), _p64(d), C.c_int64(len(r)), int(singular))

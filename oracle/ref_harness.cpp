@@ -22,6 +22,7 @@
 #include "netlist/circuit.h"
 #include "netlist/spice_parser.h"
 #include "solvers/ac_solver.h"
+#include "solvers/convergence_engine.h"
 #include "physics/device_physics.h"
 #include "tensors/kernel_dispatch.h"
 #include "tensors/physics_tensors.h"
@@ -268,6 +269,17 @@ int ref_ac(const nr_netlist* nl, const double* v_op, double f_start, double f_st
         }
     }
     return 0;
+}
+
+// ConvergenceEngine::classifyFailure on a history given as residuals and
+// voltage deltas (the fields it reads)
+int ref_classify_failure(const double* residual, const double* max_dv, int64_t n, int32_t singular) {
+    std::vector<ConvergenceStep> h((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        h[(size_t)i].residualNorm = residual[i];
+        h[(size_t)i].maxVoltageDelta = max_dv[i];
+    }
+    return (int)ConvergenceEngine::classifyFailure(h, singular != 0);
 }
 
 int ref_noise(const nr_netlist* nl, const double* v_op, int32_t output_node, double f_start, double f_stop,

@@ -712,7 +712,8 @@ def main():
             tu = bt.autotune(precision=prec, reps=3)
             tune[key] = {"s": time.perf_counter() - t0, "factor_solve_ms_before": tu["ms_before"],
                          "factor_solve_ms_after": tu["ms_after"],
-                         "buckets": [[b["rmax"] if b["rmax"] < 1 << 20 else None, b["team"], b["block"], b["fronts"]]
+                         "buckets": [[b["rmax"] if b["rmax"] < 1 << 20 else None, b["team"], b["block"], b["fronts"],
+                                      "full" if b["full"] else "panel"]
                                      for b in tu["buckets"]]}
             log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution")
     for w in range(a.warmup):

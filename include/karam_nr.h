@@ -359,10 +359,18 @@ typedef struct nr_autotune_result {
     int32_t team[NR_TUNE_BUCKETS];         /* chosen lanes per (front, instance) */
     int32_t block[NR_TUNE_BUCKETS];        /* chosen threads per block           */
     int32_t fronts[NR_TUNE_BUCKETS];       /* large fronts in the bucket         */
+    int32_t full[NR_TUNE_BUCKETS];         /* 1: the whole front in LDS          */
     double ms_before, ms_after;            /* factor + back substitution of the batch */
 } nr_autotune_result;
 NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
                              nr_autotune_result* out);
+/* Set the launch class of one front-order bucket by hand (a saved tuning
+ * table, or a fixed choice): team lanes, threads per block, full = 1 for the
+ * whole-front-in-LDS kernel.  NR_ERR_INVALID for a class that does not
+ * exist, NR_ERR_UNSUPPORTED when the bucket's fronts would not fit its LDS;
+ * like the tuner's choice it is kept by the topology and changes no result. */
+NR_API int nr_batch_set_launch_class(nr_batch* batch, int32_t precision, int32_t bucket, int32_t team,
+                                     int32_t block, int32_t full);
 
 /* Transient of every instance (the restated stepTransient): capacitor
  * companions of the instance's current method (trapezoidal G = 2C/h with

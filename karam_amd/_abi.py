@@ -187,7 +187,8 @@ NR_TUNE_BUCKETS = 8
 class NrAutotuneResult(C.Structure):
     _fields_ = [("n_buckets", C.c_int32), ("bucket_rmax", C.c_int32 * NR_TUNE_BUCKETS),
                 ("team", C.c_int32 * NR_TUNE_BUCKETS), ("block", C.c_int32 * NR_TUNE_BUCKETS),
-                ("fronts", C.c_int32 * NR_TUNE_BUCKETS), ("ms_before", C.c_double), ("ms_after", C.c_double)]
+                ("fronts", C.c_int32 * NR_TUNE_BUCKETS), ("full", C.c_int32 * NR_TUNE_BUCKETS),
+                ("ms_before", C.c_double), ("ms_after", C.c_double)]
 
 
 class NrTiming(C.Structure):

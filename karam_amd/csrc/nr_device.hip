@@ -1705,6 +1705,115 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
+// Large fronts with the whole front in LDS, one team per (front, instance):
+// the r x r front and its rhs are assembled there (the A entries, then each
+// child's update block in child order, read once in its own row-major order
+// so its cache lines are fetched once), the c pivots are eliminated
+// right-looking in LDS, and the pivot rows, the update block and the rhs are
+// streamed out in the layouts k_factor_large writes.  Every entry receives
+// the same operations in the same order as in k_factor_large (A entry, the
+// children in order, then one fma per pivot in pivot order), so results are
+// identical; only the traffic and the synchronisation differ.  For fronts
+// whose r (r + 1) values fit a team's LDS share.
+template <int TEAM, int NT, typename V>
+__global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
+                                                    const int* __restrict__ sn_list, int n_sn, int lds_team) {
+#pragma clang fp contract(off)
+    static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* smem = reinterpret_cast<V*>(smem_raw);
+    constexpr int TPB = NT / TEAM;
+    const int tib = threadIdx.x / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    constexpr int TPW = TEAM < 64 ? 64 / TEAM : 1;
+    const int nact_pad = (nact + TPW - 1) / TPW * TPW;
+    const long long task = (long long)blockIdx.x * TPB + tib;
+    if (task >= (long long)n_sn * nact_pad) return;
+    const int J = sn_list[task / nact_pad];
+    const int slot = (int)(task % nact_pad);
+    if (slot >= nact) return;
+    const int b = act[slot];
+    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
+    V* F = smem + (size_t)tib * (size_t)lds_team;     // r x r, row-major
+    V* fr = F + r * r;                                // rhs, r
+    for (int e = lane; e < r * r + r; e += TEAM) F[e] = V(0);
+    team_sync<TEAM>();
+    const int Bp = P.Bp;
+    const V* Al = vp<V>(P.Al);
+    {
+        const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
+        for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
+            V vals[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * TEAM;
+                vals[u] = (e < a1) ? Al[IX(T.fa_slot[e], b, Bp)] : V(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * TEAM;
+                if (e < a1) F[T.fa_li[e] * r + T.fa_lj[e]] += vals[u];
+            }
+        }
+    }
+    const V* rl = vp<V>(P.rhsl);
+    for (int k = lane; k < c; k += TEAM) fr[k] += rl[IX(T.perm[f + k], b, Bp)];
+    team_sync<TEAM>();
+    const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
+    for (int ch = ch0; ch < ch1; ++ch) {
+        const int C = T.children[ch];
+        const int mc = T.sn_nrows[C];
+        const int* rel = T.relmap + T.sn_row_off[C];
+        const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
+        const V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[C], b, mc, P.B);
+        const int nn = mc * mc;
+        for (int e0 = lane; e0 < nn; e0 += 8 * TEAM) {
+            V vals[8];
+            int pos[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int e = e0 + u * TEAM;
+                const int a = e < nn ? e / mc : 0, q = e < nn ? e - a * mc : 0;
+                vals[u] = (e < nn) ? cb[e] : V(0);
+                pos[u] = (e < nn) ? rel[a] * r + rel[q] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * TEAM < nn) F[pos[u]] += vals[u];
+        }
+        for (int a = lane; a < mc; a += TEAM) fr[rel[a]] += rcb[a];
+        team_sync<TEAM>();
+    }
+    int sing = 0;
+    for (int k = 0; k < c; ++k) {
+        const V piv = F[k * r + k];
+        if (vsmall(piv)) sing = 1;
+        for (int i = k + 1 + lane; i < r; i += TEAM) F[i * r + k] = F[i * r + k] / piv;
+        team_sync<TEAM>();
+        const int w = r - k - 1;
+        for (int e = lane; e < w * w; e += TEAM) {
+            const int i = k + 1 + e / w, j = k + 1 + e % w;
+            F[i * r + j] = vfma(-F[i * r + k], F[k * r + j], F[i * r + j]);
+        }
+        for (int i = k + 1 + lane; i < r; i += TEAM) fr[i] = vfma(-F[i * r + k], fr[k], fr[i]);
+        team_sync<TEAM>();
+    }
+    if (m > 0) {
+        V* cbJ = vp<V>(P.CBL) + fblk(T.sn_cb_off[J], b, m * m, P.B);
+        for (int e = lane; e < m * m; e += TEAM) {
+            const int i = e / m, j = e - i * m;
+            cbJ[e] = F[(c + i) * r + c + j];
+        }
+        V* rcbJ = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[J], b, m, P.B);
+        for (int i = lane; i < m; i += TEAM) rcbJ[i] = fr[c + i];
+    }
+    V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
+    for (int e = lane; e < c * r; e += TEAM) U[e] = F[e];
+    for (int k = lane; k < c; k += TEAM) vp<V>(P.ys)[IX(f + k, b, Bp)] = fr[k];
+    if (sing && lane == 0) P.st[b].singular = 1;
+}
+
 template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                         int nact_grid, const int* __restrict__ sn_list, int n_sn,

@@ -47,7 +47,7 @@ EXPORTED = [
     "nr_spice_parse", "nr_spice_free", "nr_spice_get_netlist", "nr_spice_get_info", "nr_spice_node_name",
     "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
     "nr_ac_opts_default", "nr_batch_ac", "nr_batch_get_ac", "nr_batch_get_ac_freqs", "nr_batch_get_ac_status",
-    "nr_batch_get_diagnostics", "nr_batch_get_trace",
+    "nr_batch_get_diagnostics", "nr_batch_get_trace", "nr_batch_set_launch_class",
     "nr_noise_opts_default", "nr_batch_noise", "nr_batch_get_noise", "nr_batch_get_noise_vrms",
     "nr_batch_get_noise_freqs", "nr_batch_get_noise_status",
 ]
@@ -83,6 +83,7 @@ def lib():
         L.nr_batch_get_ac_freqs.argtypes = [_vp, _f64p, C.c_int64]
         L.nr_batch_get_ac_status.argtypes = [_vp, C.POINTER(C.c_int32), C.c_int64]
         L.nr_batch_get_diagnostics.argtypes = [_vp, C.POINTER(_abi.NrDiagnostics), C.c_int64]
+        L.nr_batch_set_launch_class.argtypes = [_vp] + [C.c_int32] * 5
         L.nr_batch_get_trace.argtypes = [_vp, C.POINTER(_abi.NrConvStep), C.c_int64]
         L.nr_noise_opts_default.argtypes = [C.POINTER(_abi.NrNoiseOpts)]
         L.nr_noise_opts_default.restype = None
@@ -417,6 +418,11 @@ class Batch:
         self._noise_result = {k: getattr(r, k) for k, _ in _abi.NrResult._fields_}
         return freqs, psd, vrms, st
 
+    def set_launch_class(self, bucket, team, block, full=0, precision=_abi.NR_PREC_FP64):
+        """Launch class of one front-order bucket (nr_batch_set_launch_class)."""
+        _check(lib().nr_batch_set_launch_class(self.h, int(precision), int(bucket), int(team), int(block),
+                                               int(full)), self.ctx)
+
     def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
         """Choose the large-front launch shape per front-order bucket by timing
         the candidates on this batch (nr_batch_autotune); the choice is kept
@@ -424,8 +430,8 @@ class Batch:
         substitution time before and after."""
         r = _abi.NrAutotuneResult()
         _check(lib().nr_batch_autotune(self.h, int(precision), int(reps), C.byref(r)), self.ctx)
-        return {"buckets": [{"rmax": r.bucket_rmax[b], "team": r.team[b], "block": r.block[b], "fronts": r.fronts[b]}
-                            for b in range(r.n_buckets)],
+        return {"buckets": [{"rmax": r.bucket_rmax[b], "team": r.team[b], "block": r.block[b], "fronts": r.fronts[b],
+                             "full": r.full[b]} for b in range(r.n_buckets)],
                 "ms_before": r.ms_before, "ms_after": r.ms_after}
 
     def waveform(self):
@@ -485,8 +491,14 @@ class Batch:
         _check(lib().nr_batch_get_kernel_stats(self.h, None, 0, C.byref(n)), self.ctx)
         arr = (_abi.NrKernelStat * max(1, n.value))()
         _check(lib().nr_batch_get_kernel_stats(self.h, arr, n.value, C.byref(n)), self.ctx)
-        return {r.name.decode(): {"launches": r.launches, "ms": r.ms, "bytes": r.bytes, "flops": r.flops}
-                for r in arr[: n.value]}
+        out = {}
+        for r in arr[: n.value]:       # two families can share a kernel (the back substitutions): add them up
+            x = out.setdefault(r.name.decode(), {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            x["launches"] += r.launches
+            x["ms"] += r.ms
+            x["bytes"] += r.bytes
+            x["flops"] += r.flops
+        return out
 
     def close(self):
         if getattr(self, "h", None):

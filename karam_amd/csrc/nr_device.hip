@@ -1500,7 +1500,7 @@ __device__ __forceinline__ void team_sync() {
 //       or a row of the pivot columns below it (L21) and applies the c
 //       updates to it alone: no barrier;
 //   (D) the trailing m x m block (the front's update block) is formed in
-//       4x4 register tiles and streamed to HBM, its entries gathered straight
+//       4 x 2 register tiles and streamed to HBM, its entries gathered straight
 //       from the children's update blocks (every A entry of a front lies in a
 //       pivot row or column, so only children contribute).
 template <int TEAM, int NT, bool GP, typename V>
@@ -1641,55 +1641,55 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
         }
     }
     team_sync<TEAM>();
-    // (D) trailing block in 4x4 register tiles, streamed to the update block
+    // (D) trailing block in TI x TJ register tiles, streamed to the update block
     if (m > 0) {
+        constexpr int TI = 4, TJ = 2;
         V* cbJ = vp<V>(P.CBL) + fblk(T.sn_cb_off[J], b, m * m, P.B);
-        const int mt = (m + 3) >> 2;
-        for (int tt = lane; tt < mt * mt; tt += TEAM) {
-            const int i0 = (tt / mt) * 4, j0 = (tt % mt) * 4;
-            V acc[4][4];
+        const int mti = (m + TI - 1) / TI, mtj = (m + TJ - 1) / TJ;
+        for (int tt = lane; tt < mti * mtj; tt += TEAM) {
+            const int i0 = (tt / mtj) * TI, j0 = (tt % mtj) * TJ;
+            V acc[TI][TJ];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TI; ++a)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) acc[a][q] = V(0);
+                for (int q = 0; q < TJ; ++q) acc[a][q] = V(0);
             for (int ch = ch0; ch < ch1; ++ch) {
                 const int C = T.children[ch];
                 const int mc = T.sn_nrows[C];
                 const int* inv = T.inv + T.inv_off[C] + c;
                 const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
-                int ra[4], qa[4];
+                int ra[TI], qa[TJ];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
-                    ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
-                    qa[a] = (j0 + a < m) ? inv[j0 + a] : -1;
-                }
-                V x[4][4];
+                for (int a = 0; a < TI; ++a) ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int q = 0; q < TJ; ++q) qa[q] = (j0 + q < m) ? inv[j0 + q] : -1;
+                V x[TI][TJ];
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
+                for (int a = 0; a < TI; ++a)
+#pragma unroll
+                    for (int q = 0; q < TJ; ++q)
                         x[a][q] = (ra[a] >= 0 && qa[q] >= 0) ? cb[ra[a] * mc + qa[q]] : V(0);
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TI; ++a)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
+                    for (int q = 0; q < TJ; ++q)
                         if (ra[a] >= 0 && qa[q] >= 0) acc[a][q] += x[a][q];
             }
             for (int k = 0; k < c; ++k) {
-                V l[4], u[4];
+                V l[TI], u[TJ];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) l[a] = (i0 + a < m) ? Lc[(i0 + a) * ls + k] : V(0);
+                for (int a = 0; a < TI; ++a) l[a] = (i0 + a < m) ? Lc[(i0 + a) * ls + k] : V(0);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) u[q] = (j0 + q < m) ? Pr[k * r + c + j0 + q] : V(0);
+                for (int q = 0; q < TJ; ++q) u[q] = (j0 + q < m) ? Pr[k * r + c + j0 + q] : V(0);
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TI; ++a)
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) acc[a][q] = vfma(-l[a], u[q], acc[a][q]);
+                    for (int q = 0; q < TJ; ++q) acc[a][q] = vfma(-l[a], u[q], acc[a][q]);
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TI; ++a)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
+                for (int q = 0; q < TJ; ++q)
                     if (i0 + a < m && j0 + q < m) cbJ[(i0 + a) * m + j0 + q] = acc[a][q];
         }
         V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[J], b, m, P.B);

@@ -2215,26 +2215,26 @@ __global__ void k_ac_out(DevTopo T, DevBatch Q, int nact, int f, int n_freq, int
     if (p == 0 && Q.st[b].singular) status[b] = NR_STATUS_SINGULAR;
 }
 
-// Noise of frequency point f: after the solve of Y^T z = e_out (Q.xs), the
-// output noise PSD sum_k |z[pos_k] - z[neg_k]|^2 S_k over the sources in
-// computeNoiseSources order (solvers/ac_solver.h:400-504): src[k] = (kind,
-// device, pos, neg) with kind 0 a resistor (netlist index; 4kT/R), 1 a diode
-// (SoA index; 2q|I| of the clamped diode current at the operating point), 2 a
-// MOSFET (SoA index; 4kT (2/3) gm when gm > 0).  A source whose PSD is not
-// positive is left out, as the reference leaves it out of its list
-// (physics/noise_models.h); its term is stored as +0, which leaves the sum of
-// non-negative terms unchanged.  One block per 64 instances: its sixteen
-// wavefronts evaluate 256 terms at a time into LDS, and wavefront 0 adds them
-// in source order.
+// Noise sources (solvers/ac_solver.h:400-504) in computeNoiseSources order:
+// src[k] = (kind, device, pos, neg) with kind 0 a resistor (netlist index;
+// 4kT/R), 1 a diode (SoA index; 2q|I| of the clamped diode current at the
+// operating point), 2 a MOSFET (SoA index; 4kT (2/3) gm when gm > 0).  Their
+// PSDs do not depend on frequency: k_noise_psd writes them once per run,
+// interleaved (psd[IX(k, b)]), with +0 for a source whose PSD is not positive
+// (the reference leaves it out of its list, physics/noise_models.h; +0 leaves
+// the sum of non-negative terms unchanged).
 constexpr double NOISE_KB = 1.380649e-23, NOISE_QE = 1.602176634e-19;
-__device__ __forceinline__ double noise_term(const DevTopo& T, const DevBatch& P, const DevBatch& Q, int b, int4 e,
-                                             double temp_K) {
+__global__ __launch_bounds__(256) void k_noise_psd(DevTopo T, DevBatch P, int nact, const int4* __restrict__ src,
+                                                   int n_src, double temp_K, double* __restrict__ psd) {
 #pragma clang fp contract(off)
-    const int n = T.n, Bp = P.Bp;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int Bp = P.Bp;
+    if (t >= (long long)n_src * Bp) return;
+    const int k = (int)(t / Bp), b = (int)(t % Bp);
+    if (b >= nact) return;
+    const int n = T.n;
     auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
-    auto Z = [&](int node) -> cdouble {
-        return (node > 0 && node <= n) ? vp<cdouble>(Q.xs)[IX(T.iperm[node - 1], b, Bp)] : cdouble(0.0, 0.0);
-    };
+    const int4 e = src[k];
     double s;
     if (e.x == 0) {
         const double R = P.R[(size_t)b * T.nR + e.y];
@@ -2246,15 +2246,26 @@ __device__ __forceinline__ double noise_term(const DevTopo& T, const DevBatch& P
         const double gm = P.mgm[IX(e.y, b, Bp)];
         s = gm <= 0.0 ? 0.0 : 4.0 * NOISE_KB * temp_K * (2.0 / 3.0) * gm;
     }
-    if (!(s > 0.0)) return 0.0;
-    const cdouble zp = Z(e.z), zn = Z(e.w);
-    const double hr = zp.re - zn.re, hi = zp.im - zn.im;
-    return (hr * hr + hi * hi) * s;
+    psd[IX(k, b, Bp)] = s > 0.0 ? s : 0.0;
 }
 
-__global__ __launch_bounds__(HUB_SUM_THREADS) void k_noise_sum(DevTopo T, DevBatch P, DevBatch Q, int nact,
-                                                               const int4* __restrict__ src, int n_src, double temp_K,
-                                                               int f, int n_freq, double* __restrict__ psd,
+// Noise of frequency point f: after the solve of Y^T z = e_out (Q.xs), the
+// output noise PSD sum_k |z[pos_k] - z[neg_k]|^2 S_k in source order.
+// pn[k] = (pos, neg) as rows of the permuted solution (-1 for ground).  One
+// block per 64 instances: its sixteen wavefronts evaluate 256 terms at a time
+// into LDS, and wavefront 0 adds them in source order.
+__device__ __forceinline__ double noise_term(const DevBatch& Q, int b, int2 pn, double s) {
+#pragma clang fp contract(off)
+    const int Bp = Q.Bp;
+    const cdouble zp = pn.x >= 0 ? vp<cdouble>(Q.xs)[IX(pn.x, b, Bp)] : cdouble(0.0, 0.0);
+    const cdouble zn = pn.y >= 0 ? vp<cdouble>(Q.xs)[IX(pn.y, b, Bp)] : cdouble(0.0, 0.0);
+    const double hr = zp.re - zn.re, hi = zp.im - zn.im;
+    return s > 0.0 ? (hr * hr + hi * hi) * s : 0.0;
+}
+
+__global__ __launch_bounds__(HUB_SUM_THREADS) void k_noise_sum(DevBatch Q, int nact, const int2* __restrict__ pn,
+                                                               const double* __restrict__ psd, int n_src, int f,
+                                                               int n_freq, double* __restrict__ out,
                                                                int* __restrict__ status) {
 #pragma clang fp contract(off)
     __shared__ double buf[HUB_SUM_CHUNK][64];
@@ -2270,7 +2281,7 @@ __global__ __launch_bounds__(HUB_SUM_THREADS) void k_noise_sum(DevTopo T, DevBat
 #pragma unroll
         for (int u = 0; u < PER; ++u) {
             const int q = w * PER + u;
-            x[u] = q < cnt ? noise_term(T, P, Q, bb, src[base + q], temp_K) : 0.0;
+            x[u] = q < cnt ? noise_term(Q, bb, pn[base + q], psd[IX(base + q, bb, Q.Bp)]) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < PER; ++u) buf[w * PER + u][lane] = x[u];
@@ -2280,7 +2291,7 @@ __global__ __launch_bounds__(HUB_SUM_THREADS) void k_noise_sum(DevTopo T, DevBat
         __syncthreads();
     }
     if (w == 0 && on) {
-        psd[(size_t)b * n_freq + f] = total;
+        out[(size_t)b * n_freq + f] = total;
         if (Q.st[b].singular) status[b] = NR_STATUS_SINGULAR;
     }
 }

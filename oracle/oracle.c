@@ -1,9 +1,15 @@
 /*
- * oracle.c — CPU FP64 restatement of the reference NR DC path.
+ * oracle.c — CPU FP64 restatement of the reference's batched Newton-Raphson
+ * path: the DC solve (GMIN stepping, pnjlim, damping, KCL audit, convergence
+ * diagnostics), the transient (companions, fixed steps and the
+ * IntegratorArbiter's step control), the AC sweep and the noise analysis.
  *
- * TEST INFRASTRUCTURE ONLY (the checker).  Built without fast-math.  Each
- * function follows the reference file:line named in its comment, and its
- * comment states where it departs from the reference's literal behaviour.
+ * TEST INFRASTRUCTURE ONLY (the checker).  Built without fast-math and with
+ * FP contraction off.  Each function follows the reference file:line named in
+ * its comment, and its comment states where it departs from the reference's
+ * literal behaviour.  Every function is pinned by a test: golden vectors of
+ * the reference's own translation units (oracle/_ref), closed forms or
+ * brute-force checks; the pins are listed where each test file begins.
  */
 #include "oracle.h"
 

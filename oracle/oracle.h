@@ -1,6 +1,7 @@
 /*
  * oracle.h — CPU FP64 restatement of the reference engine's Newton-Raphson
- * DC path.  TEST INFRASTRUCTURE ONLY: it is the checker that tests, the smoke
+ * path (DC, transient, AC, noise).  TEST INFRASTRUCTURE ONLY: it is the
+ * checker that tests, the smoke
  * entry and the CPU-baseline leg of the benchmark compare the HIP library
  * against.  The product library never links or calls it.
  *

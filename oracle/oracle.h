@@ -1,9 +1,9 @@
 /*
  * oracle.h — CPU FP64 restatement of the reference engine's Newton-Raphson
  * path (DC, transient, AC, noise).  TEST INFRASTRUCTURE ONLY: it is the
- * checker that tests, the smoke
- * entry and the CPU-baseline leg of the benchmark compare the HIP library
- * against.  The product library never links or calls it.
+ * checker that tests, the smoke entry and the CPU-baseline leg of the
+ * benchmark compare the HIP library against.  The product library never
+ * links or calls it.
  *
  * Every function cites the reference file:line it restates.  The netlist
  * format is the library's public `nr_netlist` (include/karam_nr.h).

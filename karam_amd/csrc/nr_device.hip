@@ -86,6 +86,11 @@ static int set_err(nr_ctx* ctx, int code, const std::string& m) {
 // ---------------------------------------------------------------------------
 // device-side topology and batch views (passed by value to kernels)
 // ---------------------------------------------------------------------------
+struct alignas(16) ChildRec {
+    long long cb_off, rcb_off;   // the child's update block and rhs block
+    int nrows, pc, row_off, inv_off;
+};
+
 struct DevTopo {
     int n, nnz_a, nD, nM, nR, nV, nI;
     const int *d_a, *d_c;
@@ -117,6 +122,10 @@ struct DevTopo {
     // parent's pivot columns (a prefix, the maps are increasing), and the
     // inverse map parent-local row -> child row (or -1), per child
     const int *sn_pc, *inv_off, *inv;
+    // per child slot (children[] order): the child's fields the large-front
+    // kernels read, in one record so that one wide load, indexed by the slot
+    // alone, brings them (no dependent load through children[])
+    const struct ChildRec* child_rec;
     // fronts whose panel is kept in HBM: offset of their panel (front-major),
     // for 8-byte values and for the complex path's 16-byte values
     const long long *sn_gp_off, *sn_gp_off_c;
@@ -1564,11 +1573,11 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     team_sync<TEAM>();
     const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
     for (int ch = ch0; ch < ch1; ++ch) {
-        const int C = T.children[ch];
-        const int mc = T.sn_nrows[C], pc = T.sn_pc[C];
-        const int* rel = T.relmap + T.sn_row_off[C];
-        const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
-        const V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[C], b, mc, P.B);
+        const ChildRec cr = T.child_rec[ch];
+        const int mc = cr.nrows, pc = cr.pc;
+        const int* rel = T.relmap + cr.row_off;
+        const V* cb = vp<V>(P.CBL) + fblk(cr.cb_off, b, mc * mc, P.B);
+        const V* rcb = vp<V>(P.RCBL) + fblk(cr.rcb_off, b, mc, P.B);
         // rows a < pc (all columns) and columns q < pc of the remaining rows:
         // the child's entries that land in the panel
         const int n1 = pc * mc, nn = n1 + (mc - pc) * pc;
@@ -1654,10 +1663,10 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
 #pragma unroll
                 for (int q = 0; q < TJ; ++q) acc[a][q] = V(0);
             for (int ch = ch0; ch < ch1; ++ch) {
-                const int C = T.children[ch];
-                const int mc = T.sn_nrows[C];
-                const int* inv = T.inv + T.inv_off[C] + c;
-                const V* cb = vp<V>(P.CBL) + fblk(T.sn_cb_off[C], b, mc * mc, P.B);
+                const ChildRec cr = T.child_rec[ch];
+                const int mc = cr.nrows;
+                const int* inv = T.inv + cr.inv_off + c;
+                const V* cb = vp<V>(P.CBL) + fblk(cr.cb_off, b, mc * mc, P.B);
                 int ra[TI], qa[TJ];
 #pragma unroll
                 for (int a = 0; a < TI; ++a) ra[a] = (i0 + a < m) ? inv[i0 + a] : -1;

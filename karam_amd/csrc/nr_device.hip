@@ -86,6 +86,14 @@ static int set_err(nr_ctx* ctx, int code, const std::string& m) {
 // ---------------------------------------------------------------------------
 // device-side topology and batch views (passed by value to kernels)
 // ---------------------------------------------------------------------------
+// One large front of a launch list: its fields the large-front kernels
+// read, in one record (the list entry itself), so that no load depends on a
+// supernode index first
+struct alignas(16) FrontRec {
+    long long u_off, cb_off, rcb_off;
+    int c, m, f, a_begin, a_end, ch_begin, ch_end, row_off, J, pad;
+};
+
 struct alignas(16) ChildRec {
     long long cb_off, rcb_off;   // the child's update block and rhs block
     int nrows, pc, row_off, inv_off;
@@ -1514,7 +1522,7 @@ __device__ __forceinline__ void team_sync() {
 //       pivot row or column, so only children contribute).
 template <int TEAM, int NT, bool GP, typename V>
 __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
-                                                     const int* __restrict__ sn_list, int n_sn, int lds_team) {
+                                                     const FrontRec* __restrict__ fr_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
     // a team wider than a wavefront synchronises the whole block, so it must
     // be alone in it (early exits and per-front barrier counts then agree)
@@ -1533,11 +1541,12 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const int nact_pad = (nact + TPW - 1) / TPW * TPW;
     const long long task = (long long)blockIdx.x * TPB + tib;
     if (task >= (long long)n_sn * nact_pad) return;
-    const int J = sn_list[task / nact_pad];
+    const FrontRec fe = fr_list[task / nact_pad];
+    const int J = fe.J;
     const int slot = (int)(task % nact_pad);
     if (slot >= nact) return;
     const int b = act[slot];
-    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
+    const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
     const int np = c * r + m * c + r;                 // panel size
     V* Pr = GP ? vp<V>(P.gpanel) + fblk(gp_offsets<V>(T)[J], b, np, P.B)
                : smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
@@ -1553,7 +1562,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const int Bp = P.Bp;
     const V* Al = vp<V>(P.Al);
     {
-        const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
+        const int a0 = fe.a_begin, a1 = fe.a_end;
         for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
             V vals[8];
 #pragma unroll
@@ -1571,7 +1580,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     const V* rl = vp<V>(P.rhsl);
     for (int k = lane; k < c; k += TEAM) fr[k] += rl[IX(T.perm[f + k], b, Bp)];
     team_sync<TEAM>();
-    const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
+    const int ch0 = fe.ch_begin, ch1 = fe.ch_end;
     for (int ch = ch0; ch < ch1; ++ch) {
         const ChildRec cr = T.child_rec[ch];
         const int mc = cr.nrows, pc = cr.pc;
@@ -1653,7 +1662,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     // (D) trailing block in TI x TJ register tiles, streamed to the update block
     if (m > 0) {
         constexpr int TI = 4, TJ = 2;
-        V* cbJ = vp<V>(P.CBL) + fblk(T.sn_cb_off[J], b, m * m, P.B);
+        V* cbJ = vp<V>(P.CBL) + fblk(fe.cb_off, b, m * m, P.B);
         const int mti = (m + TI - 1) / TI, mtj = (m + TJ - 1) / TJ;
         for (int tt = lane; tt < mti * mtj; tt += TEAM) {
             const int i0 = (tt / mtj) * TI, j0 = (tt % mtj) * TJ;
@@ -1701,14 +1710,14 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
                 for (int q = 0; q < TJ; ++q)
                     if (i0 + a < m && j0 + q < m) cbJ[(i0 + a) * m + j0 + q] = acc[a][q];
         }
-        V* rcb = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[J], b, m, P.B);
+        V* rcb = vp<V>(P.RCBL) + fblk(fe.rcb_off, b, m, P.B);
         for (int i = lane; i < m; i += TEAM) {
             V acc = fr[c + i];
             for (int k = 0; k < c; ++k) acc = vfma(-Lc[i * ls + k], fr[k], acc);
             rcb[i] = acc;
         }
     }
-    V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
+    V* U = vp<V>(P.UL) + fblk(fe.u_off, b, c * r, P.B);
     for (int e = lane; e < c * r; e += TEAM) U[e] = Pr[e];
     for (int k = lane; k < c; k += TEAM) vp<V>(P.ys)[IX(f + k, b, Bp)] = fr[k];
     if (sing && lane == 0) P.st[b].singular = 1;
@@ -1726,7 +1735,7 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
 // whose r (r + 1) values fit a team's LDS share.
 template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
-                                                    const int* __restrict__ sn_list, int n_sn, int lds_team) {
+                                                    const FrontRec* __restrict__ fr_list, int n_sn, int lds_team) {
 #pragma clang fp contract(off)
     static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
     const int nact = nact_grid > 0 ? nact_grid : *P.nact;
@@ -1739,11 +1748,11 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
     const int nact_pad = (nact + TPW - 1) / TPW * TPW;
     const long long task = (long long)blockIdx.x * TPB + tib;
     if (task >= (long long)n_sn * nact_pad) return;
-    const int J = sn_list[task / nact_pad];
+    const FrontRec fe = fr_list[task / nact_pad];
     const int slot = (int)(task % nact_pad);
     if (slot >= nact) return;
     const int b = act[slot];
-    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
+    const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
     V* F = smem + (size_t)tib * (size_t)lds_team;     // r x r, row-major
     V* fr = F + r * r;                                // rhs, r
     for (int e = lane; e < r * r + r; e += TEAM) F[e] = V(0);
@@ -1751,7 +1760,7 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
     const int Bp = P.Bp;
     const V* Al = vp<V>(P.Al);
     {
-        const int a0 = T.sn_a_begin[J], a1 = T.sn_a_end[J];
+        const int a0 = fe.a_begin, a1 = fe.a_end;
         for (int e0 = a0 + lane; e0 < a1; e0 += 8 * TEAM) {
             V vals[8];
 #pragma unroll
@@ -1769,7 +1778,7 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
     const V* rl = vp<V>(P.rhsl);
     for (int k = lane; k < c; k += TEAM) fr[k] += rl[IX(T.perm[f + k], b, Bp)];
     team_sync<TEAM>();
-    const int ch0 = T.sn_child_begin[J], ch1 = T.sn_child_end[J];
+    const int ch0 = fe.ch_begin, ch1 = fe.ch_end;
     for (int ch = ch0; ch < ch1; ++ch) {
         const int C = T.children[ch];
         const int mc = T.sn_nrows[C];
@@ -1809,15 +1818,15 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
         team_sync<TEAM>();
     }
     if (m > 0) {
-        V* cbJ = vp<V>(P.CBL) + fblk(T.sn_cb_off[J], b, m * m, P.B);
+        V* cbJ = vp<V>(P.CBL) + fblk(fe.cb_off, b, m * m, P.B);
         for (int e = lane; e < m * m; e += TEAM) {
             const int i = e / m, j = e - i * m;
             cbJ[e] = F[(c + i) * r + c + j];
         }
-        V* rcbJ = vp<V>(P.RCBL) + fblk(T.sn_rcb_off[J], b, m, P.B);
+        V* rcbJ = vp<V>(P.RCBL) + fblk(fe.rcb_off, b, m, P.B);
         for (int i = lane; i < m; i += TEAM) rcbJ[i] = fr[c + i];
     }
-    V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
+    V* U = vp<V>(P.UL) + fblk(fe.u_off, b, c * r, P.B);
     for (int e = lane; e < c * r; e += TEAM) U[e] = F[e];
     for (int k = lane; k < c; k += TEAM) vp<V>(P.ys)[IX(f + k, b, Bp)] = fr[k];
     if (sing && lane == 0) P.st[b].singular = 1;
@@ -1825,7 +1834,7 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
 
 template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
-                                                        int nact_grid, const int* __restrict__ sn_list, int n_sn,
+                                                        int nact_grid, const FrontRec* __restrict__ fr_list, int n_sn,
                                                         int rmax) {
 #pragma clang fp contract(off)
     // a team wider than a wavefront synchronises the whole block, so it must
@@ -1845,15 +1854,15 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     const int nact_pad = (nact + TPW - 1) / TPW * TPW;
     const long long task = (long long)blockIdx.x * TPB + tib;
     if (task >= (long long)n_sn * nact_pad) return;
-    const int J = sn_list[task / nact_pad];
+    const FrontRec fe = fr_list[task / nact_pad];
     const int slot = (int)(task % nact_pad);
     if (slot >= nact) return;
     const int b = act[slot];
     const int Bp = P.Bp;
-    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
+    const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
     V* xl = smem + (size_t)tib * (size_t)rmax;
-    const V* U = vp<V>(P.UL) + fblk(T.sn_u_off[J], b, c * r, P.B);
-    const int* rows = T.row_idx + T.sn_row_off[J];
+    const V* U = vp<V>(P.UL) + fblk(fe.u_off, b, c * r, P.B);
+    const int* rows = T.row_idx + fe.row_off;
     V* xs = vp<V>(P.xs);
     const V* ys = vp<V>(P.ys);
     for (int a = lane; a < m; a += TEAM) xl[c + a] = xs[IX(rows[a], b, Bp)];

@@ -94,6 +94,13 @@ struct alignas(16) FrontRec {
     int c, m, f, a_begin, a_end, ch_begin, ch_end, row_off, J, pad;
 };
 
+// One small front of a unit, in the unit's walk order: its fields the
+// lane-per-instance kernels read, in one record
+struct alignas(16) SmallRec {
+    long long u_off, cb_off, rcb_off;
+    int c, m, f, nch, tb_off, root, row_off, J;
+};
+
 struct alignas(16) ChildRec {
     long long cb_off, rcb_off;   // the child's update block and rhs block
     int nrows, pc, row_off, inv_off;
@@ -1353,13 +1360,13 @@ __device__ __forceinline__ size_t fblk(long long off, int b, int size, int B) {
 // item per front position (the pool's zero row where nothing contributes),
 // so every load is unconditional and the sum order matches the oracle's.
 template <int R, typename V>
-__device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, int J, int b, bool on) {
+__device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, const SmallRec& e, int b, bool on) {
 #pragma clang fp contract(off)
     constexpr int TB = R * R + R;
     const int Bp = P.Bp;
-    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    const int nch = T.sn_child_end[J] - T.sn_child_begin[J];
-    const int* tb = T.tb + T.tb_off[J];
+    const int c = e.c, m = e.m, r = c + m, f = e.f;
+    const int nch = e.nch;
+    const int* tb = T.tb + e.tb_off;
     const V* pool = vp<V>(P.pool);
     const int bb = on ? b : 0;
     V F[R * R];
@@ -1393,8 +1400,8 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
         }
     }
     if (on) {
-        const long long uo = T.sn_u_off[J], co = T.sn_cb_off[J], ro = T.sn_rcb_off[J];
-        const bool root = T.sn_root[J] != 0;
+        const long long uo = e.u_off, co = e.cb_off, ro = e.rcb_off;
+        const bool root = e.root != 0;
         V* cbl = vp<V>(P.CBL) + fblk(co, b, m * m, P.B);
         V* rcbl = vp<V>(P.RCBL) + fblk(ro, b, m, P.B);
 #pragma unroll
@@ -1421,12 +1428,12 @@ __device__ __forceinline__ int front_small(const DevTopo& T, const DevBatch& P, 
 }
 
 template <int R, typename V>
-__device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, int J, int b, bool on) {
+__device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, const SmallRec& e, int b, bool on) {
 #pragma clang fp contract(off)
     const int Bp = P.Bp;
-    const int c = T.sn_ncols[J], m = T.sn_nrows[J], r = c + m, f = T.sn_first[J];
-    const long long uo = T.sn_u_off[J];
-    const int* rows = T.row_idx + T.sn_row_off[J];
+    const int c = e.c, m = e.m, r = c + m, f = e.f;
+    const long long uo = e.u_off;
+    const int* rows = T.row_idx + e.row_off;
     const int bb = on ? b : 0;
     V t[R];
 #pragma unroll
@@ -1454,7 +1461,7 @@ __device__ __forceinline__ void back_small(const DevTopo& T, const DevBatch& P, 
 template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
                                                       const int* __restrict__ sub_ptr,
-                                                      const int* __restrict__ sub_sn, int n_sub) {
+                                                      const SmallRec* __restrict__ sub_rec, int n_sub) {
     // eager launches pass the active count; graph replays pass minus the
     // grid's count and read the live one (at most that) from the device
     const int nact = nact_grid > 0 ? nact_grid : *P.nact;
@@ -1467,14 +1474,14 @@ __global__ __launch_bounds__(256) void k_factor_small(DevTopo T, DevBatch P, con
     const bool on = slot < nact;
     const int b = on ? act[slot] : 0;
     int sing = 0;
-    for (int q = sub_ptr[sub]; q < sub_ptr[sub + 1]; ++q) sing |= front_small<RMAX, V>(T, P, sub_sn[q], b, on);
+    for (int q = sub_ptr[sub]; q < sub_ptr[sub + 1]; ++q) sing |= front_small<RMAX, V>(T, P, sub_rec[q], b, on);
     if (on && sing) P.st[b].singular = 1;
 }
 
 template <int RMAX, typename V>
 __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                          int nact_grid, const int* __restrict__ sub_ptr,
-                                                         const int* __restrict__ sub_sn, int n_sub) {
+                                                         const SmallRec* __restrict__ sub_rec, int n_sub) {
     // eager launches pass the active count; graph replays pass minus the
     // grid's count and read the live one (at most that) from the device
     const int nact = nact_grid > 0 ? nact_grid : *P.nact;
@@ -1486,7 +1493,7 @@ __global__ __launch_bounds__(256) void k_backsolve_small(DevTopo T, DevBatch P, 
     const int slot = g * 64 + lane;
     const bool on = slot < nact;
     const int b = on ? act[slot] : 0;
-    for (int q = sub_ptr[sub + 1] - 1; q >= sub_ptr[sub]; --q) back_small<RMAX, V>(T, P, sub_sn[q], b, on);
+    for (int q = sub_ptr[sub + 1] - 1; q >= sub_ptr[sub]; --q) back_small<RMAX, V>(T, P, sub_rec[q], b, on);
 }
 
 // ---------------------------------------------------------------------------

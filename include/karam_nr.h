@@ -232,7 +232,13 @@ NR_API int nr_topology_export(const nr_topology* topo, int32_t what,
 
 /* ---- batches -------------------------------------------------------------- */
 /* B instances of one topology, each with its own parameters (MC/corners).
- * Device buffers are allocated here once; solves allocate nothing. */
+ * Device buffers are allocated here once; solves allocate nothing beyond
+ * buffers a first use sizes (the dense LU's workspaces, traces, waveforms).
+ * A topology's device tables belong to the context of its first batch:
+ * batches of it in another context are refused (NR_ERR_INVALID); build one
+ * topology per context.  A topology's launch classes (nr_batch_autotune,
+ * nr_batch_set_launch_class) are shared by all its batches, so those calls
+ * must not run while another thread solves a batch of the same topology. */
 NR_API int nr_batch_alloc(nr_ctx* ctx, const nr_topology* topo, int32_t B,
                           nr_batch** out);
 NR_API int nr_batch_free(nr_batch* batch);
@@ -252,6 +258,24 @@ NR_API int nr_batch_clear_seed(nr_batch* batch);
  * flow, iteration accounting and statuses as the FP64 path. */
 enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
 
+/* Linear-solve route of each Newton step.  The reference's solveDC solves
+ * with the dense partial-pivoting LU solveLU_Pivoted (math/linalg.h:742-854,
+ * routed at solvers/circuitsim_solver.cpp:291-295).
+ *  NR_LU_AUTO         multifrontal LU on the reference's Min-Fill ordering
+ *                     (no pivoting), then each solve's relative residual
+ *                     max|b - A x| / max|b| in FP64; a solve above 1e-8
+ *                     (FP32 path: 1e-3 on its float system), not finite or
+ *                     with a pivot below 1e-25 is redone by the restated
+ *                     solveLU_Pivoted (FP32 path: in FP64 on the float
+ *                     system).  Circuits above NR_DENSE_MAX_N nodes take
+ *                     the multifrontal solve alone.
+ *  NR_LU_MULTIFRONTAL the multifrontal LU alone.
+ *  NR_LU_DENSE        the restated solveLU_Pivoted for every solve: the
+ *                     reference's LU bit for bit, O(n^3) per Newton step;
+ *                     n <= NR_DENSE_MAX_N, else NR_ERR_UNSUPPORTED. */
+enum { NR_LU_AUTO = 0, NR_LU_MULTIFRONTAL = 1, NR_LU_DENSE = 2 };
+#define NR_DENSE_MAX_N 4096
+
 typedef struct nr_solve_opts {
     int32_t max_iter;        /* per GMIN step, reference MAX_NR_ITER = 50     */
     int32_t gmin_steps;      /* reference GMIN_STEPS = 19                     */
@@ -267,6 +291,7 @@ typedef struct nr_solve_opts {
                                 GMIN-stepping start                           */
     int32_t trace_capacity;  /* > 0: keep each instance's last trace_capacity
                                 ConvergenceStep records (nr_batch_get_trace) */
+    int32_t lu;              /* NR_LU_*: the linear-solve route              */
 } nr_solve_opts;
 NR_API void nr_solve_opts_default(nr_solve_opts* opts);
 
@@ -337,6 +362,7 @@ typedef struct nr_transient_opts {
     int32_t wave_decimation; /* record every k-th accepted step (>= 1), as
                                 recordWaveform (shaders/gpu_nr_loop.wgsl:
                                 862-891)                                    */
+    int32_t lu;              /* NR_LU_*, as in nr_solve_opts                 */
 } nr_transient_opts;
 NR_API void nr_transient_opts_default(nr_transient_opts* opts);
 
@@ -494,6 +520,8 @@ typedef struct nr_instance_stats {
     double kcl;                  /* final max |KCL| (A), FP64                */
     int32_t seeded;              /* 1: the result comes from the seed; 0: from
                                     the plain start (no seed, or fallback)   */
+    int32_t lu_fallbacks;        /* NR_LU_AUTO: Newton steps of the last solve
+                                    that the dense pivoting LU solved        */
 } nr_instance_stats;
 NR_API int nr_batch_get_stats(const nr_batch* batch, nr_instance_stats* out,
                               int64_t count);

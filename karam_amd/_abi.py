@@ -35,6 +35,9 @@ NR_EXPORT_MOS_ORDER = 8
 
 NR_PREC_FP64 = 0
 NR_PREC_FP32_REFINE = 1
+# linear-solve routes (nr_solve_opts.lu / nr_transient_opts.lu)
+NR_LU_AUTO, NR_LU_MULTIFRONTAL, NR_LU_DENSE = 0, 1, 2
+NR_DENSE_MAX_N = 4096
 
 NR_STATUS_CONVERGED = 0
 NR_STATUS_NOT_CONVERGED = 1
@@ -108,7 +111,7 @@ class NrSolveOpts(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("precision", C.c_int32),
         ("seeded_start_step", C.c_int32), ("record_timing", C.c_int32), ("use_graph", C.c_int32),
-        ("fallback_plain", C.c_int32), ("trace_capacity", C.c_int32),
+        ("fallback_plain", C.c_int32), ("trace_capacity", C.c_int32), ("lu", C.c_int32),
     ]
 
 
@@ -136,7 +139,7 @@ class NrTransientOpts(C.Structure):
         ("h", C.c_double), ("steps", C.c_int32), ("max_iter", C.c_int32), ("start", C.c_int32),
         ("n_probe", C.c_int32), ("probe", C.POINTER(C.c_int32)), ("record_timing", C.c_int32),
         ("precision", C.c_int32), ("use_graph", C.c_int32), ("method", C.c_int32), ("control", C.c_int32),
-        ("t_stop", C.c_double), ("wave_capacity", C.c_int32), ("wave_decimation", C.c_int32),
+        ("t_stop", C.c_double), ("wave_capacity", C.c_int32), ("wave_decimation", C.c_int32), ("lu", C.c_int32),
     ]
 
 
@@ -169,7 +172,7 @@ class NrInstanceStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32), ("ref_iterations", C.c_int32), ("status", C.c_int32),
         ("gmin_steps_done", C.c_int32), ("kcl", C.c_double),
-        ("seeded", C.c_int32),
+        ("seeded", C.c_int32), ("lu_fallbacks", C.c_int32),
     ]
 
 

@@ -149,12 +149,17 @@ struct DevTopo {
     int nC;
     const int *cap_n1, *cap_n2, *vsrc_wave, *isrc_wave;
     const double *vsrc_wave_par, *isrc_wave_par;
+    // CSR pattern of A (original node order): the residual check and the
+    // dense LU read the assembled values row by row through slot_dst
+    const int *row_ptr, *col_idx;
 };
 
+// lu_fb: Newton steps of the instance that the dense LU solved (NR_LU_AUTO)
 struct InstState {
     int step, it, total, ref;
     int status, done, singular, seeded;
     double prev_kcl, kcl;
+    int lu_fb, pad_;
 };
 
 // Per-instance transient step control: the instance's time, the step and
@@ -235,6 +240,20 @@ struct DevBatch {
     InstState* st;
     int* act;
     int* nact;
+    // linear-solve route (NR_LU_*).  AUTO: per instance, the bits of
+    // max |b - A x| and max |b| (non-negative doubles order as their bits)
+    // and a non-finite flag, gathered by k_linres and cleared by k_lu_flag;
+    // the list of instances the dense LU solves, with dn_ctl[0] = their count
+    // and dn_ctl[1] = the head of the queue its workgroups take them from;
+    // dn_slots workspaces of n * n doubles (null when the route does not use
+    // the dense LU)
+    int lu;
+    int dn_slots;
+    unsigned long long* dn_acc;
+    int* dn_bad;
+    int* dn_list;
+    int* dn_ctl;
+    double* dn_mat;
 };
 
 __device__ __forceinline__ size_t IX(long long item, int b, int Bp) { return (size_t)item * (size_t)Bp + (size_t)b; }
@@ -748,6 +767,7 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         s.status = NR_STATUS_NOT_CONVERGED; s.done = mode != 0 ? DONE_ARMED : DONE_RUNNING; s.singular = 0;
         s.seeded = use_seed;
         s.prev_kcl = 0.0; s.kcl = 0.0;
+        s.lu_fb = 0; s.pad_ = 0;
         P.st[b] = s;
         if (mode == 0) {
             DiagState d;
@@ -1945,6 +1965,285 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
             for (int rr = w; rr < 64; rr += 4)
                 if (row0 + rr < n) dst[IX(row0 + rr, b, Bp)] = tile[rr][lane];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernels: the pivoting fallback of the linear solve
+// ---------------------------------------------------------------------------
+// The multifrontal LU does not pivot.  Where a circuit's Jacobian needs row
+// exchanges for stability (a long inverter chain: each stage's gm exceeds
+// its output conductance by the stage gain, and eliminating the chain in
+// order multiplies the supply column by that gain at every stage), its
+// element growth leaves a solution whose residual is orders of magnitude
+// above the right-hand side.  NR_LU_AUTO measures every solve's relative
+// residual max|b - A x| / max|b| in FP64 and hands the instances above the
+// bound to the dense LU with partial pivoting that the reference runs
+// (solveLU_Pivoted, math/linalg.h:742-854, reached from solvers/
+// circuitsim_solver.cpp:291-295); NR_LU_DENSE uses that LU for every solve.
+constexpr int LINRES_ROWS = 16;          // rows per wavefront of k_linres
+constexpr double LU_ETA_FP64 = 1e-8;     // bound of the FP64 path's ratio
+constexpr double LU_ETA_FP32 = 1e-3;     // bound of the FP32 path's float system
+constexpr int DENSE_THREADS = 1024;
+
+// NR_LU_AUTO, after the back substitution: per instance, max_i |b_i - (A x)_i|
+// and max_i |b_i| from the assembled values (the factorisation leaves the A
+// entries and the rhs in place) and the solution xs, every row's residual
+// accumulated in CSR order in FP64.  Lane = instance; each wavefront covers
+// LINRES_ROWS rows, the block's four are combined in LDS and merged into the
+// instance's maxima with one atomic each.
+template <typename V>
+__global__ __launch_bounds__(256) void k_linres(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid) {
+#pragma clang fp contract(off)
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    __shared__ double smr[4][64], smb[4][64];
+    __shared__ int sbad[4][64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int slot = blockIdx.y * 64 + lane;
+    const bool on = slot < nact;
+    const int b = on ? act[slot] : 0, Bp = P.Bp;
+    double mr = 0.0, mb = 0.0;
+    int bad = 0;
+    if (on) {
+        const V* pool = vp<V>(P.pool);
+        const V* Al = vp<V>(P.Al);
+        const V* rl = vp<V>(P.rhsl);
+        const V* xs = vp<V>(P.xs);
+        const int i0 = (blockIdx.x * 4 + w) * LINRES_ROWS, i1 = min(i0 + LINRES_ROWS, T.n);
+        for (int i = i0; i < i1; ++i) {
+            const int dn = T.node_dst[i];
+            const double bi = (double)((dn & 1) ? rl[IX(dn >> 1, b, Bp)] : pool[IX(dn >> 1, b, Bp)]);
+            double r = bi;
+            for (int q = T.row_ptr[i]; q < T.row_ptr[i + 1]; ++q) {
+                const int d = T.slot_dst[q];
+                const double a = (double)((d & 1) ? Al[IX(d >> 1, b, Bp)] : pool[IX(d >> 1, b, Bp)]);
+                r = r - a * (double)xs[IX(T.iperm[T.col_idx[q]], b, Bp)];
+            }
+            const double xi = (double)xs[IX(T.iperm[i], b, Bp)];
+            if (!isfinite(r) || !isfinite(xi) || !isfinite(bi)) bad = 1;
+            if (fabs(r) > mr) mr = fabs(r);
+            if (fabs(bi) > mb) mb = fabs(bi);
+        }
+    }
+    smr[w][lane] = mr;
+    smb[w][lane] = mb;
+    sbad[w][lane] = bad;
+    __syncthreads();
+    if (w == 0 && on && blockIdx.x * 4 * LINRES_ROWS < T.n) {
+        for (int u = 1; u < 4; ++u) {
+            if (smr[u][lane] > mr) mr = smr[u][lane];
+            if (smb[u][lane] > mb) mb = smb[u][lane];
+            bad |= sbad[u][lane];
+        }
+        // non-negative doubles order as their bit patterns
+        atomicMax(&P.dn_acc[2 * (size_t)b], (unsigned long long)__double_as_longlong(mr));
+        atomicMax(&P.dn_acc[2 * (size_t)b + 1], (unsigned long long)__double_as_longlong(mb));
+        if (bad) atomicOr(&P.dn_bad[b], 1);
+    }
+}
+
+// NR_LU_AUTO: list the active instances whose solve the dense LU redoes (a
+// value that is not finite, a residual above bound * max|b|, or a pivot the
+// multifrontal LU found below 1e-25), clear the maxima for the next sweep and
+// rewind the queue.  One block.
+__global__ __launch_bounds__(1024) void k_lu_flag(DevBatch P, const int* __restrict__ act, double bound) {
+    __shared__ int cnt;
+    const int nact = *P.nact;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    for (int s = threadIdx.x; s < nact; s += blockDim.x) {
+        const int b = act[s];
+        const double mr = __longlong_as_double((long long)P.dn_acc[2 * (size_t)b]);
+        const double mb = __longlong_as_double((long long)P.dn_acc[2 * (size_t)b + 1]);
+        const int bad = P.dn_bad[b];
+        P.dn_acc[2 * (size_t)b] = 0ull;
+        P.dn_acc[2 * (size_t)b + 1] = 0ull;
+        P.dn_bad[b] = 0;
+        if (bad || !(mr <= bound * mb) || P.st[b].singular) P.dn_list[atomicAdd(&cnt, 1)] = b;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        P.dn_ctl[0] = cnt;
+        P.dn_ctl[1] = 0;
+    }
+}
+
+// NR_LU_DENSE: every active instance goes to the dense LU.  One block.
+__global__ __launch_bounds__(1024) void k_dense_all(DevBatch P, const int* __restrict__ act) {
+    const int nact = *P.nact;
+    for (int s = threadIdx.x; s < nact; s += blockDim.x) P.dn_list[s] = act[s];
+    if (threadIdx.x == 0) {
+        P.dn_ctl[0] = nact;
+        P.dn_ctl[1] = 0;
+    }
+}
+
+// larger |value|, then the lower row: the pivot rule of solveLU_Pivoted
+__device__ __forceinline__ void piv_better(double& v, int& k, double v2, int k2) {
+    if (v2 > v || (v2 == v && k2 < k)) { v = v2; k = k2; }
+}
+
+// solveLU_Pivoted (math/linalg.h:742-854) on the listed instances, one
+// workgroup per instance taken from the queue, `dn_slots` workgroups.  The
+// system is the assembled A (as double) and rhs in the original node order,
+// dense and row-major in the workgroup's workspace.  Every element receives
+// the reference's operations in the reference's order: for each pivot step
+// the pivot is the largest |a| in the column (NaN never, ties to the lower
+// row; below 1e-25 the system is singular), whole rows and rhs entries are
+// swapped, each lower row's multiplier is a / pivot and, when not zero, its
+// trailing entries take a - l * u; the forward substitution applies the
+// columns to each rhs entry in increasing order, and the backward
+// substitution subtracts each row's products in increasing column order
+// before dividing by the diagonal.  So the solution is bit-identical to the
+// sequential routine.  It is written to xm in the layout k_update reads
+// (permuted rows).  On the FP32 path the system is the FP64 one: every A
+// entry summed as k_assemble sums it (linear part, FP64 device conductances
+// in term order, the gmin diagonal) and the FP64 Newton residual, for the
+// correction; rounding an ill-conditioned Jacobian to float alone moves its
+// Newton step.  count_fb: add one to the instance's fallback count.
+// Dynamic LDS: 3 n doubles.
+__device__ double slot_value64(const DevTopo& T, const DevBatch& P, int b, int q, double gmin) {
+#pragma clang fp contract(off)
+    const int n = T.n, Bp = P.Bp;
+    auto V = [&](int node) -> double { return (node > 0 && node <= n) ? P.vI[IX(node - 1, b, Bp)] : 0.0; };
+    double a = P.glin[IX(q, b, Bp)];
+    for (int k = T.nl_ptr[q]; k < T.nl_ptr[q + 1]; ++k) {
+        const int t = T.nl_term[k];
+        const int kind = term_kind(t), i = term_idx(t);
+        double g;
+        if (kind == karam::SK_DIODE) {
+            double id;
+            diode_eval(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], T.d_vt[i], id, g);
+        } else {
+            double vgs, vds, ids, gm, gds;
+            mos_eval(V(T.m_g[i]), V(T.m_d[i]), V(T.m_s[i]), T.m_w[i], T.m_l[i], P.kpI[IX(i, b, Bp)],
+                     P.vthI[IX(i, b, Bp)], T.m_lam[i], T.m_pmos[i], vgs, vds, ids, gm, gds);
+            g = kind == karam::SK_MOS_GDS ? gds : gm;
+        }
+        a = term_neg(t) ? a + (-g) : a + g;
+    }
+    if (T.slot_diag[q]) a = a + gmin;
+    return a;
+}
+
+template <typename V>
+__global__ __launch_bounds__(DENSE_THREADS) void k_dense(DevTopo T, DevBatch P, int count_fb,
+                                                         const double* __restrict__ gmin_tab, int gmin_last) {
+#pragma clang fp contract(off)
+    extern __shared__ double dsm[];
+    __shared__ int s_q, s_k;
+    __shared__ double s_v;
+    __shared__ double rv[DENSE_THREADS / 64];
+    __shared__ int rk[DENSE_THREADS / 64];
+    const int n = T.n, tid = threadIdx.x, nt = blockDim.x, Bp = P.Bp;
+    const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+    double* srow = dsm;            // pivot row (factor); products (backward)
+    double* sl = dsm + n;          // multipliers (factor); x (backward)
+    double* sy = dsm + 2 * n;      // rhs -> y
+    double* M = P.dn_mat + (size_t)blockIdx.x * (size_t)n * (size_t)n;
+    const V* pool = vp<V>(P.pool);
+    const V* Al = vp<V>(P.Al);
+    const V* rl = vp<V>(P.rhsl);
+    for (;;) {
+        if (tid == 0) s_q = atomicAdd(&P.dn_ctl[1], 1);
+        __syncthreads();
+        const int q = s_q;
+        const int cnt = P.dn_ctl[0];
+        __syncthreads();
+        if (q >= cnt) return;
+        const int b = P.dn_list[q];
+        for (size_t e = tid; e < (size_t)n * (size_t)n; e += nt) M[e] = 0.0;
+        __syncthreads();
+        if (P.p32) {
+            const double gmin = gmin_tab[min(P.st[b].step, gmin_last)];
+            for (int i = tid; i < n; i += nt) {
+                for (int k = T.row_ptr[i]; k < T.row_ptr[i + 1]; ++k)
+                    M[(size_t)i * n + T.col_idx[k]] = 0.0 + slot_value64(T, P, b, k, gmin);
+                sy[i] = P.rhslin[IX(i, b, Bp)] - gmin * P.vI[IX(i, b, Bp)];
+            }
+        } else {
+            for (int i = tid; i < n; i += nt) {
+                for (int k = T.row_ptr[i]; k < T.row_ptr[i + 1]; ++k) {
+                    const int d = T.slot_dst[k];
+                    const double a = (double)((d & 1) ? Al[IX(d >> 1, b, Bp)] : pool[IX(d >> 1, b, Bp)]);
+                    M[(size_t)i * n + T.col_idx[k]] = 0.0 + a;
+                }
+                const int dn = T.node_dst[i];
+                sy[i] = (double)((dn & 1) ? rl[IX(dn >> 1, b, Bp)] : pool[IX(dn >> 1, b, Bp)]);
+            }
+        }
+        __syncthreads();
+        int sing = 0;
+        for (int i = 0; i < n; ++i) {
+            double bv = 0.0;
+            int bk = 0x7fffffff;
+            for (int k = i + tid; k < n; k += nt) {
+                const double v = fabs(M[(size_t)k * n + i]);
+                if (v > bv) { bv = v; bk = k; }
+            }
+            for (int o = 32; o > 0; o >>= 1) piv_better(bv, bk, __shfl_xor(bv, o), __shfl_xor(bk, o));
+            if (lane == 0) { rv[wv] = bv; rk[wv] = bk; }
+            __syncthreads();
+            if (tid == 0) {
+                double v = rv[0];
+                int k = rk[0];
+                for (int u = 1; u < nw; ++u) piv_better(v, k, rv[u], rk[u]);
+                s_v = v;
+                s_k = v > 0.0 ? k : i;
+            }
+            __syncthreads();
+            const int p = s_k;
+            if (s_v < 1e-25) { sing = 1; break; }
+            if (p != i) {
+                for (int c = tid; c < n; c += nt) {
+                    const double tmp = M[(size_t)i * n + c];
+                    M[(size_t)i * n + c] = M[(size_t)p * n + c];
+                    M[(size_t)p * n + c] = tmp;
+                }
+                if (tid == 0) { const double tb = sy[i]; sy[i] = sy[p]; sy[p] = tb; }
+                __syncthreads();
+            }
+            const double piv = M[(size_t)i * n + i];
+            for (int c = i + 1 + tid; c < n; c += nt) srow[c] = M[(size_t)i * n + c];
+            for (int j = i + 1 + tid; j < n; j += nt) {
+                const double l = M[(size_t)j * n + i] / piv;
+                M[(size_t)j * n + i] = l;
+                sl[j] = l;
+            }
+            __syncthreads();
+            for (int j = i + 1 + wv; j < n; j += nw) {
+                const double l = sl[j];
+                if (l != 0.0) {
+                    double* rj = M + (size_t)j * n;
+                    for (int c = i + 1 + lane; c < n; c += 64) rj[c] = rj[c] - l * srow[c];
+                }
+            }
+            __syncthreads();
+        }
+        if (!sing) {
+            for (int j = 0; j < n; ++j) {
+                const double yj = sy[j];
+                for (int i = j + 1 + tid; i < n; i += nt) sy[i] = sy[i] - M[(size_t)i * n + j] * yj;
+                __syncthreads();
+            }
+            for (int i = n - 1; i >= 0; --i) {
+                for (int j = i + 1 + tid; j < n; j += nt) srow[j] = M[(size_t)i * n + j] * sl[j];
+                __syncthreads();
+                if (tid == 0) {
+                    double xi = sy[i];
+                    for (int j = i + 1; j < n; ++j) xi = xi - srow[j];
+                    sl[i] = xi / M[(size_t)i * n + i];
+                }
+                __syncthreads();
+            }
+        }
+        double* xm = P.xm + (size_t)b * n;
+        for (int i = tid; i < n; i += nt) xm[T.iperm[i]] = sing ? 0.0 : sl[i];
+        if (tid == 0) {
+            P.st[b].singular = sing;
+            if (count_fb) P.st[b].lu_fb += 1;
+        }
+        __syncthreads();
     }
 }
 

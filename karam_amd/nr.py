@@ -283,12 +283,12 @@ class Topology:
 
 
 def solve_opts(max_iter=50, gmin_steps=19, precision=_abi.NR_PREC_FP64, seeded_start_step=18,
-               record_timing=0, use_graph=0, fallback_plain=1, trace_capacity=0):
+               record_timing=0, use_graph=0, fallback_plain=1, trace_capacity=0, lu=_abi.NR_LU_AUTO):
     o = _abi.NrSolveOpts()
     lib().nr_solve_opts_default(C.byref(o))
     o.max_iter, o.gmin_steps, o.precision = max_iter, gmin_steps, precision
     o.seeded_start_step, o.record_timing, o.use_graph = seeded_start_step, record_timing, use_graph
-    o.fallback_plain, o.trace_capacity = fallback_plain, trace_capacity
+    o.fallback_plain, o.trace_capacity, o.lu = fallback_plain, trace_capacity, int(lu)
     return o
 
 
@@ -347,7 +347,7 @@ class Batch:
 
     def transient(self, h=1e-9, steps=200, max_iter=50, start=_abi.NR_TRAN_START_ZERO, probes=None,
                   record_timing=0, precision=_abi.NR_PREC_FP64, use_graph=0, method=_abi.NR_INT_GEAR1,
-                  control=_abi.NR_TRAN_FIXED, t_stop=0.0, wave_capacity=0, wave_decimation=1):
+                  control=_abi.NR_TRAN_FIXED, t_stop=0.0, wave_capacity=0, wave_decimation=1, lu=_abi.NR_LU_AUTO):
         """Transient of every instance (needs a transient topology): `steps`
         fixed steps of h, or with control=NR_TRAN_ARBITER the reference's
         step control up to t_stop.  probes: 1-based node ids recorded in the
@@ -363,6 +363,7 @@ class Batch:
         o.use_graph = int(use_graph)
         o.method, o.control, o.t_stop = int(method), int(control), float(t_stop)
         o.wave_capacity, o.wave_decimation = int(wave_capacity), int(wave_decimation)
+        o.lu = int(lu)
         r = _abi.NrResult()
         _check(lib().nr_batch_transient(self.h, C.byref(o), C.byref(r)), self.ctx)
         ring = int(wave_capacity)
@@ -476,6 +477,7 @@ class Batch:
             "gmin_steps_done": np.array([x.gmin_steps_done for x in arr]),
             "kcl": np.array([x.kcl for x in arr]),
             "seeded": np.array([x.seeded for x in arr]),
+            "lu_fallbacks": np.array([x.lu_fallbacks for x in arr]),
         }
 
     def timing(self):

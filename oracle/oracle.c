@@ -1208,6 +1208,42 @@ int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float*
     return rc;
 }
 
+/* The product's default route (NR_LU_AUTO).  The multifrontal LU does not
+ * pivot; on circuits whose Jacobian needs row exchanges for stability (long
+ * inverter chains, where a stage's gm exceeds its output conductance by the
+ * stage gain) its element growth leaves a solution whose residual is many
+ * orders above rounding level.  Such a solve is redone by the dense LU with
+ * partial pivoting the reference runs (solveLU_Pivoted, math/linalg.h:
+ * 742-854, reached from solvers/circuitsim_solver.cpp:291-295).  The test is
+ * the relative residual max_i |b_i - (A x)_i| <= bound * max_i |b_i|, every
+ * row's residual accumulated in CSR order, so that the HIP check reaches the
+ * same decision.  (A normwise backward error, residual over |A| |x| + |b|,
+ * would pass a float solve that has grown to 1e16 V: it is small whenever x
+ * is large.) */
+int orc_lu_flag(const orc_topo* t, const double* a_vals, const double* rhs, const double* x, double bound,
+                double* eta) {
+    const int32_t n = t->n;
+    double mr = 0.0, mb = 0.0;
+    int bad = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        double r = rhs[i];
+        for (int32_t k = t->row_ptr[i]; k < t->row_ptr[i + 1]; ++k) r = r - a_vals[k] * x[t->col_idx[k]];
+        if (!isfinite(r) || !isfinite(x[i]) || !isfinite(rhs[i])) bad = 1;
+        if (fabs(r) > mr) mr = fabs(r);
+        if (fabs(rhs[i]) > mb) mb = fabs(rhs[i]);
+    }
+    if (eta) *eta = bad ? INFINITY : mr / mb;
+    return bad || !(mr <= bound * mb);
+}
+
+int orc_lu_auto(const orc_topo* t, const double* a_vals, const double* rhs, double* x, int32_t* fallback) {
+    if (fallback) *fallback = 0;
+    int rc = orc_lu_multifrontal(t, a_vals, rhs, x, NULL);
+    if (rc == NR_OK && !orc_lu_flag(t, a_vals, rhs, x, ORC_LU_ETA_FP64, NULL)) return NR_OK;
+    if (fallback) *fallback = 1;
+    return orc_lu_dense(t->n, t->row_ptr, t->col_idx, a_vals, rhs, x, NULL);
+}
+
 static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v, double* res, const tran_ctx* tc);
 
 int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, double gmin, float* a32) {
@@ -1225,16 +1261,55 @@ int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, d
  * (FP64 update).  The Newton residual b - A v of the reference's Norton
  * system is the KCL audit vector minus the gmin diagonal current. */
 static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_topo* t, const double* v,
-                           double gmin, const tran_ctx* tc, double* vals, double* lin, float* a32, float* r32,
-                           float* x32, double* res, double* vn) {
+                           double gmin, const tran_ctx* tc, int32_t lu_kind, double* vals, double* lin, float* a32,
+                           float* r32, float* x32, double* res, double* vn, int32_t* fallback) {
     const int32_t n = nl->num_nodes;
+    *fallback = 0;
     int rc = assemble_f32(nl, s, t, v, gmin, tc, vals, lin, a32);
     if (rc != NR_OK) return rc;
     kcl_soa(nl, s, v, res, tc);
     for (int32_t i = 0; i < n; ++i) r32[i] = (float)(res[i] - gmin * v[i]);
-    rc = orc_lu_multifrontal_f32(t, a32, r32, x32);
-    for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + (double)x32[i];
+    int dense = (lu_kind == ORC_LU_DENSE);
+    if (!dense) {
+        rc = orc_lu_multifrontal_f32(t, a32, r32, x32);
+        for (int32_t i = 0; i < n; ++i) vn[i] = (double)x32[i];
+        if (lu_kind == ORC_LU_AUTO) {
+            double* ad = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
+            double* rd = (double*)xmalloc(sizeof(double) * (size_t)n);
+            for (int32_t k = 0; k < t->nnz_a; ++k) ad[k] = (double)a32[k];
+            for (int32_t i = 0; i < n; ++i) rd[i] = (double)r32[i];
+            dense = rc != NR_OK || orc_lu_flag(t, ad, rd, vn, ORC_LU_ETA_FP32, NULL);
+            *fallback = dense;
+            free(ad); free(rd);
+        }
+    }
+    if (dense) {
+        /* the dense LU solves the Newton correction with the FP64 Jacobian
+         * (the FP64 path's A) and the FP64 residual: rounding the Jacobian
+         * to float alone moves the step of an ill-conditioned system */
+        double* rhs64 = (double*)xmalloc(sizeof(double) * (size_t)n);
+        rc = stamp_with_soa(nl, s, t, v, gmin, vals, rhs64, tc);
+        if (rc == NR_OK) {
+            orc_assemble(t, vals, lin);
+            for (int32_t i = 0; i < n; ++i) rhs64[i] = res[i] - gmin * v[i];
+            rc = orc_lu_dense(n, t->row_ptr, t->col_idx, lin, rhs64, vn, NULL);
+        }
+        free(rhs64);
+    }
+    for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + vn[i];
     return rc;
+}
+
+/* one FP64 Newton system solved as `lu_kind` says */
+static int solve_f64(const orc_topo* t, int32_t lu_kind, const double* av, const double* rhs, double* x,
+                     int32_t* fallback) {
+    *fallback = 0;
+    switch (lu_kind) {
+        case ORC_LU_DENSE: return orc_lu_dense(t->n, t->row_ptr, t->col_idx, av, rhs, x, NULL);
+        case ORC_LU_SPARSE: return orc_lu_sparse(t, av, rhs, x, NULL);
+        case ORC_LU_AUTO: return orc_lu_auto(t, av, rhs, x, fallback);
+        default: return orc_lu_multifrontal(t, av, rhs, x, NULL);
+    }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1420,7 +1495,7 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 /* a15: solveDC (solvers/circuitsim_solver.cpp:176-555)                     */
 /* ------------------------------------------------------------------------ */
 void orc_dc_opts_default(orc_dc_opts* o) {
-    o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_MULTIFRONTAL;
+    o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_AUTO;
     o->start_step = 0; o->use_seed = 0; o->precision = ORC_PREC_FP64;
     o->trace_cap = 0; o->pad = 0; o->trace = NULL;
 }
@@ -1514,16 +1589,15 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         int32_t it = 0;
         for (; it < o.max_iter; ++it) {
             int rc;
+            int32_t fb = 0;
             if (p32) {
-                rc = newton_step_f32(nl, s, t, v, gmin, NULL, vals, av, a32, r32, x32, res, vn);
+                rc = newton_step_f32(nl, s, t, v, gmin, NULL, o.lu_kind, vals, av, a32, r32, x32, res, vn, &fb);
             } else {
                 stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
                 orc_assemble(t, vals, av);
-                rc = (o.lu_kind == ORC_LU_DENSE)
-                         ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
-                         : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
-                                                        : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+                rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb);
             }
+            out->lu_fallbacks += fb;
             if (rc != NR_OK) {
                 final_kcl = 1e9;
                 singular = 1; sing_step = 1;
@@ -1676,7 +1750,7 @@ double orc_wave_value(int32_t kind, const double* par, double dc, double time) {
 }
 
 void orc_tran_opts_default(orc_tran_opts* o) {
-    o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_MULTIFRONTAL;
+    o->h = 1e-9; o->steps = 200; o->max_iter = 50; o->start = 0; o->lu_kind = ORC_LU_AUTO;
     o->precision = ORC_PREC_FP64; o->method = ORC_INT_GEAR1; o->control = ORC_TRAN_FIXED; o->t_stop = 0.0;
     o->wave_cap = 0; o->decimation = 1;
 }
@@ -1903,16 +1977,15 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
             int32_t it = 0;
             for (; it < o.max_iter; ++it) {
                 int rc;
+                int32_t fb = 0;
                 if (p32) {
-                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, vals, av, a32, r32, x32, res, vn);
+                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, o.lu_kind, vals, av, a32, r32, x32, res, vn, &fb);
                 } else {
                     stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
                     orc_assemble(t, vals, av);
-                    rc = (o.lu_kind == ORC_LU_DENSE)
-                             ? orc_lu_dense(n, t->row_ptr, t->col_idx, av, rhs, vn, NULL)
-                             : (o.lu_kind == ORC_LU_SPARSE) ? orc_lu_sparse(t, av, rhs, vn, NULL)
-                                                            : orc_lu_multifrontal(t, av, rhs, vn, NULL);
+                    rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb);
                 }
+                out->lu_fallbacks += fb;
                 if (rc != NR_OK) { sing = 1; break; }
                 for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
                 kcl = kcl_soa(nl, s, vn, res, &tc);

@@ -73,6 +73,22 @@ int orc_lu_multifrontal(const orc_topo* t, const double* a_vals, const double* r
                         double* x, double* smallest_pivot);
 /* The same in float (FP32 path). */
 int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float* rhs, float* x);
+/* The linear-solve route of the product's default (NR_LU_AUTO): the
+ * multifrontal LU, then its relative residual in FP64,
+ *   max_i |b_i - (A x)_i| <= bound * max_i |b_i|,
+ * each row's residual taken in CSR order.  orc_lu_flag returns 1 when the
+ * solve is to be redone by the dense partial-pivoting LU: a value that is not
+ * finite, or a residual above the bound (ORC_LU_ETA_FP64 for the FP64 path,
+ * ORC_LU_ETA_FP32 for the FP32 path's float system).  *eta (optional): the
+ * ratio, or infinity when a value is not finite. */
+#define ORC_LU_ETA_FP64 1e-8
+#define ORC_LU_ETA_FP32 1e-3
+int orc_lu_flag(const orc_topo* t, const double* a_vals, const double* rhs, const double* x, double bound,
+                double* eta);
+/* NR_LU_AUTO on one FP64 system: multifrontal, flagged solves (and singular
+ * ones) redone by orc_lu_dense.  *fallback (optional) is set to 1 when the
+ * dense LU gave the result. */
+int orc_lu_auto(const orc_topo* t, const double* a_vals, const double* rhs, double* x, int32_t* fallback);
 /* FP32 path: the float Jacobian (CSR values) of the DC system at v. */
 int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, double gmin, float* a32);
 
@@ -88,7 +104,10 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
                           double kcl, double kcl_tol);
 
 /* --- a15: solveDC ---------------------------------------------------------- */
-enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2 };
+/* ORC_LU_AUTO: orc_lu_auto (the product's default).  On the FP32 path,
+ * ORC_LU_DENSE and ORC_LU_AUTO solve the float system with orc_lu_dense in
+ * FP64 (always, or when orc_lu_flag flags the float multifrontal solve). */
+enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2, ORC_LU_AUTO = 3 };
 /* ORC_PREC_FP64: the reference's FP64 path.  ORC_PREC_FP32_REFINE: the
  * Jacobian (device evaluation, stamps, assembly) and the multifrontal LU in
  * float; each Newton step solves J dv = b - A v with the Newton residual
@@ -105,7 +124,7 @@ typedef struct orc_conv_step {
 typedef struct orc_dc_opts {
     int32_t max_iter;     /* 50 */
     int32_t gmin_steps;   /* 19 */
-    int32_t lu_kind;      /* ORC_LU_* (FP64 path) */
+    int32_t lu_kind;      /* ORC_LU_* */
     int32_t start_step;   /* GMIN step to start at (seeded start) */
     int32_t use_seed;     /* 1: v_inout holds the start vector */
     int32_t precision;    /* ORC_PREC_* */
@@ -126,7 +145,7 @@ typedef struct orc_dc_out {
     int32_t stagnation, divergence;
     int32_t worst_node;      /* 1-based, largest |KCL| at the result  */
     int32_t gmin_used;       /* the last GMIN value is below 1e-3     */
-    int32_t pad;
+    int32_t lu_fallbacks;    /* Newton steps the dense LU solved (ORC_LU_AUTO) */
     double rms;              /* RMS of the per-node KCL at the result */
     double worst_v;          /* the worst node's voltage              */
 } orc_dc_out;
@@ -158,7 +177,7 @@ typedef struct orc_tran_opts {
     int32_t steps;        /* fixed control: steps to take                 */
     int32_t max_iter;     /* 50                                           */
     int32_t start;        /* 0: zero state, 1: v_io holds the state */
-    int32_t lu_kind;      /* ORC_LU_* (FP64 path)                         */
+    int32_t lu_kind;      /* ORC_LU_*                                     */
     int32_t precision;    /* ORC_PREC_*                                   */
     int32_t method;       /* ORC_INT_* at t = 0                           */
     int32_t control;      /* ORC_TRAN_*                                   */
@@ -179,6 +198,8 @@ typedef struct orc_tran_out {
     double kcl;           /* KCL of the last Newton iterate (A)           */
     double t_end;         /* time of the last accepted step               */
     double h_last;        /* step of the last attempt                     */
+    int32_t lu_fallbacks; /* Newton steps the dense LU solved (ORC_LU_AUTO) */
+    int32_t pad;
 } orc_tran_out;
 void orc_tran_opts_default(orc_tran_opts* o);
 /* resolved waveform ring capacity of `o` */

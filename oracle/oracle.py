@@ -41,7 +41,8 @@ class OrcTranOut(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("status", C.c_int32), ("steps_done", C.c_int32),
                 ("rejected", C.c_int32), ("method", C.c_int32), ("switched_at", C.c_int32),
                 ("instability", C.c_int32), ("taa_rejections", C.c_int32), ("recorded", C.c_int64),
-                ("kcl", C.c_double), ("t_end", C.c_double), ("h_last", C.c_double)]
+                ("kcl", C.c_double), ("t_end", C.c_double), ("h_last", C.c_double),
+                ("lu_fallbacks", C.c_int32), ("pad", C.c_int32)]
 
 
 class OrcAcOpts(C.Structure):
@@ -63,7 +64,7 @@ class OrcDcOut(C.Structure):
                 ("iters_per_step", C.c_int32 * 32),
                 ("history", C.c_int64), ("failure", C.c_int32), ("stagnation", C.c_int32),
                 ("divergence", C.c_int32), ("worst_node", C.c_int32), ("gmin_used", C.c_int32),
-                ("pad", C.c_int32), ("rms", C.c_double), ("worst_v", C.c_double)]
+                ("lu_fallbacks", C.c_int32), ("rms", C.c_double), ("worst_v", C.c_double)]
 
 
 FAIL_NONE, FAIL_SINGULAR, FAIL_DIVERGENCE, FAIL_LIMIT_CYCLE, FAIL_SLOW, FAIL_FLOATING, FAIL_OVERFLOW = range(7)
@@ -72,6 +73,7 @@ FAIL_NONE, FAIL_SINGULAR, FAIL_DIVERGENCE, FAIL_LIMIT_CYCLE, FAIL_SLOW, FAIL_FLO
 LU_DENSE = 0
 LU_SPARSE = 1
 LU_MULTIFRONTAL = 2
+LU_AUTO = 3          # multifrontal, redone by the dense pivoting LU when its residual is large
 
 PREC_FP64 = 0
 PREC_FP32_REFINE = 1
@@ -176,7 +178,7 @@ class Topology:
             self.h = None
 
 
-def dc_opts(max_iter=50, gmin_steps=19, lu=LU_MULTIFRONTAL, start_step=0, use_seed=0, precision=PREC_FP64,
+def dc_opts(max_iter=50, gmin_steps=19, lu=LU_AUTO, start_step=0, use_seed=0, precision=PREC_FP64,
             trace_cap=0, trace=None):
     return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision, trace_cap, 0, trace)
 
@@ -196,7 +198,7 @@ def _trace_dict(tr):
     return {k: np.array(a[k]) for k in ("residual", "max_dv", "damping", "gmin_step", "iteration", "worst_node")}
 
 
-def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, gmin_steps=19,
+def solve_dc(nl, topo: Topology | None = None, lu=LU_AUTO, max_iter=50, gmin_steps=19,
              seed_v=None, start_step=0, precision=PREC_FP64):
     c, keep = nl.to_c()
     v = np.zeros(nl.num_nodes) if seed_v is None else np.array(seed_v, dtype=np.float64)
@@ -206,7 +208,7 @@ def solve_dc(nl, topo: Topology | None = None, lu=LU_MULTIFRONTAL, max_iter=50, 
     return v, out
 
 
-def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL, threads=0,
+def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_AUTO, threads=0,
                    max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0, trace_cap=0):
     """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array;
     seeds (optional, [B, n]) start every instance at GMIN step start_step.
@@ -243,6 +245,7 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_MULTIFRONTAL,
         "worst_v": np.array([x.worst_v for x in outs]),
         "rms": np.array([x.rms for x in outs]),
         "gmin_used": np.array([x.gmin_used for x in outs]),
+        "lu_fallbacks": np.array([x.lu_fallbacks for x in outs]),
     }
     if tr is not None:
         stats["trace"] = {k: x.reshape(B, trace_cap) for k, x in _trace_dict(tr).items()}
@@ -275,7 +278,7 @@ def arbiter_analyze(nl, cur, hv0, hv1, dt, prev, prev2, prev_energy, prev_dt, in
 
 
 def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, steps=200, max_iter=50, start=0,
-                          v0=None, probes=None, lu=LU_MULTIFRONTAL, threads=0, precision=PREC_FP64,
+                          v0=None, probes=None, lu=LU_AUTO, threads=0, precision=PREC_FP64,
                           method=INT_GEAR1, control=TRAN_FIXED, t_stop=0.0, wave_capacity=0, decimation=1,
                           full=False):
     """Transient of B instances; returns (v_final [B, n], wave [B, cap, n_probe],
@@ -304,7 +307,7 @@ def solve_transient_batch(nl, topo: Topology, fields: dict, B: int, h=1e-9, step
         raise ValueError(f"orc_solve_transient_batch: invalid arguments (status {rc})")
     stats = {k: np.array([getattr(x, k) for x in outs])
              for k in ("iterations", "status", "steps_done", "rejected", "method", "switched_at", "instability",
-                       "taa_rejections", "recorded", "kcl", "t_end", "h_last")}
+                       "taa_rejections", "recorded", "kcl", "t_end", "h_last", "lu_fallbacks")}
     if full:
         return v, wave, stats, times
     return v, wave, stats
@@ -434,6 +437,29 @@ def lu_multifrontal(topo: Topology, a_vals, rhs):
     rc = L.orc_lu_multifrontal(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
                                _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(sp))
     return rc, x
+
+
+def lu_flag(topo: Topology, a_vals, rhs, x, bound=1e-8):
+    """(flagged, eta): the AUTO route's test of a solve x, eta = max|b - A x| / max|b|."""
+    L = lib()
+    L.orc_lu_flag.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, C.c_double, C.POINTER(C.c_double)]
+    w = C.c_double()
+    f = L.orc_lu_flag(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
+                      _p64(np.ascontiguousarray(rhs, np.float64)), _p64(np.ascontiguousarray(x, np.float64)),
+                      bound, C.byref(w))
+    return bool(f), w.value
+
+
+def lu_auto(topo: Topology, a_vals, rhs):
+    """(status, x, fallback): the multifrontal LU, redone by the dense
+    pivoting LU when its residual is above 1e-8 of max|b| or it is singular."""
+    L = lib()
+    L.orc_lu_auto.argtypes = [C.c_void_p, _f64p, _f64p, _f64p, C.POINTER(C.c_int32)]
+    x = np.zeros(topo.n)
+    fb = C.c_int32()
+    rc = L.orc_lu_auto(topo.h, _p64(np.ascontiguousarray(a_vals, np.float64)),
+                       _p64(np.ascontiguousarray(rhs, np.float64)), _p64(x), C.byref(fb))
+    return rc, x, int(fb.value)
 
 
 def lu_multifrontal_f32(topo: Topology, a_vals, rhs):

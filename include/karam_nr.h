@@ -673,6 +673,24 @@ NR_API int nr_eval_mosfets(nr_ctx* ctx, int32_t count,
                            const double* voltages, int32_t n_voltages,
                            double* vgs, double* vds, double* ids,
                            double* gm, double* gds);
+/* nr_eval_mosfets without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  The device copies of the inputs
+ * and outputs live in a fixed 8 MiB array on each GPU, held until the process
+ * ends: a call may stage at most 8 MiB (4 int32 and 10 double arrays of
+ * `count`, plus the voltages, each padded to 256 bytes: about 80,000
+ * MOSFETs with as many voltages); larger calls return NR_ERR_INVALID.  Calls
+ * on one GPU share that array: the library serialises them, and the results
+ * equal nr_eval_mosfets' bit for bit. */
+NR_API int nr_eval_mosfets_noalloc(nr_ctx* ctx, int32_t count,
+                                   const int32_t* drains, const int32_t* gates,
+                                   const int32_t* sources,
+                                   const double* W, const double* L, const double* Kp,
+                                   const double* Vth, const double* lambda,
+                                   const int32_t* isPMOS,
+                                   const double* voltages, int32_t n_voltages,
+                                   double* vgs, double* vds, double* ids,
+                                   double* gm, double* gds);
 
 #ifdef __cplusplus
 }

@@ -43,6 +43,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <random>
 #include <string>
 #include <utility>
@@ -2705,6 +2706,15 @@ __global__ void k_eval_mos_slot(int count, const int* d, const int* g, const int
     mos_eval(volt(v, nv, g[i]), volt(v, nv, d[i]), volt(v, nv, s[i]), W[i], L[i], Kp[i], Vth[i], lam[i], pm[i],
              vgs[i], vds[i], ids[i], gm[i], gds[i]);
 }
+
+// Staging of the no-allocation MOSFET evaluation slot (nr_eval_mosfets_noalloc):
+// one call's inputs and outputs.  Every GPU the library runs on holds its own
+// copy until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device (kept per device by
+// nr_ctx_create), never through this name, which on the host is not a device
+// address.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
 
 #include "nr_host.inc"
 #include "seeder.inc"

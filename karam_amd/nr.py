@@ -38,7 +38,7 @@ EXPORTED = [
     "nr_batch_alloc", "nr_batch_free", "nr_batch_set_params", "nr_batch_set_seed",
     "nr_batch_clear_seed", "nr_solve_opts_default", "nr_batch_solve_dc",
     "nr_batch_get_voltages", "nr_batch_get_stats", "nr_batch_get_timing", "nr_batch_get_kernel_stats",
-    "nr_eval_diodes", "nr_eval_mosfets",
+    "nr_eval_diodes", "nr_eval_mosfets", "nr_eval_mosfets_noalloc",
     "nr_transient_opts_default", "nr_batch_transient", "nr_batch_get_waveform", "nr_batch_get_waveform_times",
     "nr_batch_get_wave_recorded", "nr_batch_get_tran_stats", "nr_batch_autotune",
     "nr_seeder_create", "nr_seeder_destroy", "nr_seeder_record", "nr_seeder_history", "nr_seeder_reset",
@@ -139,6 +139,7 @@ def lib():
                                      _f64p, _f64p, _f64p]
         L.nr_eval_mosfets.argtypes = [_vp, C.c_int32, _i32p, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                       _i32p, _f64p, C.c_int32, _f64p, _f64p, _f64p, _f64p, _f64p]
+        L.nr_eval_mosfets_noalloc.argtypes = L.nr_eval_mosfets.argtypes
         _LIB = L
     return _LIB
 

@@ -266,8 +266,9 @@ enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
  *                     max|b - A x| / max|b| in FP64; a solve above 1e-8
  *                     (FP32 path: 1e-3 on its float system), not finite or
  *                     with a pivot below 1e-25 is redone by the restated
- *                     solveLU_Pivoted (FP32 path: in FP64 on the float
- *                     system).  Circuits above NR_DENSE_MAX_N nodes take
+ *                     solveLU_Pivoted (FP32 path: on the FP64 Jacobian and
+ *                     the FP64 Newton residual).  Circuits above
+ *                     NR_DENSE_MAX_N nodes take
  *                     the multifrontal solve alone.
  *  NR_LU_MULTIFRONTAL the multifrontal LU alone.
  *  NR_LU_DENSE        the restated solveLU_Pivoted for every solve: the

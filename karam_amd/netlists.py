@@ -298,14 +298,16 @@ def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
 
 def random_logic(n_target: int = 20000, seed: int = 1, n_inputs: int = 32, window: int = 3, width: int = 32,
                  spread: int = 2, p_inv: float = 0.3, max_fanout: int = 4, vdd: float = 1.8,
-                 cload: float = 10e-15, n_pulse: int = 8) -> Netlist:
-    """Levelised random CMOS logic of NAND2 and INV gates (level-1 MOSFETs,
+                 cload: float = 10e-15, n_pulse: int = 8, p_nand3: float = 0.0) -> Netlist:
+    """Levelised random CMOS logic of NAND2, NAND3 and INV gates (level-1 MOSFETs,
     the inverter-chain cards).  Node 1 is VDD; then n_inputs primary inputs
     driven by DC sources at 0 or VDD (seeded); every gate output has a
     `cload` capacitor to ground.  Gate inputs are drawn from the nets of the
     previous `window` levels (locality), each net driving at most
     `max_fanout` gates.  A NAND2 is two series NMOS (with an internal node)
-    and two parallel PMOS; an INV is one NMOS and one PMOS.  Gates are added
+    and two parallel PMOS; a NAND3 (probability p_nand3) three series NMOS
+    (two internal nodes) and three parallel PMOS; an INV is one NMOS and one
+    PMOS, so every gate brings two MOSFETs per node.  Gates are added
     until the node count reaches n_target.  Each level has `width` gate
     slots; slot p draws its inputs from slots p - spread .. p + spread of the
     previous `window` levels, which keeps the circuit banded (its
@@ -345,11 +347,18 @@ def random_logic(n_target: int = 20000, seed: int = 1, n_inputs: int = 32, windo
         for p in range(width):
             if node >= n_target:
                 break
-            if rng.random() < p_inv:
+            r = rng.random()
+            if r < p_inv:
                 (a,) = pick(1, p)
                 out = node + 1; node = out
                 nmos(out, a, 0); pmos(out, a, 1)
                 fan[a] += 1
+            elif r < p_inv + p_nand3:
+                a, b, c = pick(3, p)
+                out, m1, m2 = node + 1, node + 2, node + 3; node = m2
+                nmos(out, a, m1); nmos(m1, b, m2); nmos(m2, c, 0)
+                pmos(out, a, 1); pmos(out, b, 1); pmos(out, c, 1)
+                fan[a] += 1; fan[b] += 1; fan[c] += 1
             else:
                 a, b = pick(2, p)
                 out, mid = node + 1, node + 2; node = mid

@@ -795,7 +795,25 @@ static int cmp_key_idx(const void* a, const void* b) {
 
 int orc_topology_build(const nr_netlist* nl, orc_topo** out) { return orc_topology_build_ex(nl, 0, out); }
 
+static int topology_build(const nr_netlist* nl, int32_t transient, const int32_t* ordering, int32_t ordering_len,
+                          orc_topo** out);
+
 int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** out) {
+    return topology_build(nl, transient, NULL, 0, out);
+}
+
+/* The same topology with the raw Min-Fill ordering given (an earlier run's
+ * NR_EXPORT_ORDERING_RAW of the same netlist), so that the literal Min-Fill,
+ * O(n^2 d^2), need not run again.  The treewidth is taken from the fill:
+ * the largest number of entries right of the diagonal in a row of U. */
+int orc_topology_build_ordered(const nr_netlist* nl, int32_t transient, const int32_t* ordering,
+                               int32_t ordering_len, orc_topo** out) {
+    if (!ordering || ordering_len <= 0) return NR_ERR_INVALID;
+    return topology_build(nl, transient, ordering, ordering_len, out);
+}
+
+static int topology_build(const nr_netlist* nl, int32_t transient, const int32_t* ordering, int32_t ordering_len,
+                          orc_topo** out) {
     if (!nl || nl->num_nodes <= 0) return NR_ERR_INVALID;
     orc_topo* t = (orc_topo*)xcalloc(1, sizeof(orc_topo));
     t->n = nl->num_nodes;
@@ -832,9 +850,23 @@ int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** ou
     for (int32_t i = 0; i < n; ++i) t->row_ptr[i + 1] += t->row_ptr[i];
     t->nnz_a = nnz;
     free(keys); free(idx);
-    min_fill(nl, t->soa, t);
+    if (ordering) {
+        t->ordering_len = t->n_vars = ordering_len;
+        t->ordering_raw = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(ordering_len + 1));
+        memcpy(t->ordering_raw, ordering, sizeof(int32_t) * (size_t)ordering_len);
+    } else {
+        min_fill(nl, t->soa, t);
+    }
     project_ordering(t);
     symbolic(t);
+    if (ordering) {
+        int32_t tw = 0;
+        for (int32_t i = 0; i < n; ++i) {
+            const int32_t right = t->frow_ptr[i + 1] - t->fdiag[i] - 1;
+            if (right > tw) tw = right;
+        }
+        t->treewidth = tw;
+    }
     supernodes(t);
     *out = t;
     return NR_OK;
@@ -1239,6 +1271,8 @@ int orc_lu_flag(const orc_topo* t, const double* a_vals, const double* rhs, cons
 int orc_lu_auto(const orc_topo* t, const double* a_vals, const double* rhs, double* x, int32_t* fallback) {
     if (fallback) *fallback = 0;
     int rc = orc_lu_multifrontal(t, a_vals, rhs, x, NULL);
+    /* above NR_DENSE_MAX_N nodes the route is the multifrontal solve alone */
+    if (t->n > NR_DENSE_MAX_N) return rc;
     if (rc == NR_OK && !orc_lu_flag(t, a_vals, rhs, x, ORC_LU_ETA_FP64, NULL)) return NR_OK;
     if (fallback) *fallback = 1;
     return orc_lu_dense(t->n, t->row_ptr, t->col_idx, a_vals, rhs, x, NULL);
@@ -1273,7 +1307,7 @@ static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_top
     if (!dense) {
         rc = orc_lu_multifrontal_f32(t, a32, r32, x32);
         for (int32_t i = 0; i < n; ++i) vn[i] = (double)x32[i];
-        if (lu_kind == ORC_LU_AUTO) {
+        if (lu_kind == ORC_LU_AUTO && t->n <= NR_DENSE_MAX_N) {
             double* ad = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
             double* rd = (double*)xmalloc(sizeof(double) * (size_t)n);
             for (int32_t k = 0; k < t->nnz_a; ++k) ad[k] = (double)a32[k];

@@ -86,8 +86,9 @@ int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float*
 int orc_lu_flag(const orc_topo* t, const double* a_vals, const double* rhs, const double* x, double bound,
                 double* eta);
 /* NR_LU_AUTO on one FP64 system: multifrontal, flagged solves (and singular
- * ones) redone by orc_lu_dense.  *fallback (optional) is set to 1 when the
- * dense LU gave the result. */
+ * ones) redone by orc_lu_dense; above NR_DENSE_MAX_N nodes the multifrontal
+ * solve alone.  *fallback (optional) is set to 1 when the dense LU gave the
+ * result. */
 int orc_lu_auto(const orc_topo* t, const double* a_vals, const double* rhs, double* x, int32_t* fallback);
 /* FP32 path: the float Jacobian (CSR values) of the DC system at v. */
 int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, double gmin, float* a32);
@@ -105,8 +106,9 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 
 /* --- a15: solveDC ---------------------------------------------------------- */
 /* ORC_LU_AUTO: orc_lu_auto (the product's default).  On the FP32 path,
- * ORC_LU_DENSE and ORC_LU_AUTO solve the float system with orc_lu_dense in
- * FP64 (always, or when orc_lu_flag flags the float multifrontal solve). */
+ * ORC_LU_DENSE (always) and ORC_LU_AUTO (when orc_lu_flag flags the float
+ * multifrontal solve) solve the Newton correction with orc_lu_dense on the
+ * FP64 Jacobian and the FP64 Newton residual. */
 enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2, ORC_LU_AUTO = 3 };
 /* ORC_PREC_FP64: the reference's FP64 path.  ORC_PREC_FP32_REFINE: the
  * Jacobian (device evaluation, stamps, assembly) and the multifrontal LU in
@@ -166,6 +168,10 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
 /* --- a16: transient (stepTransient restated) -------------------------------- */
 /* Pattern including the capacitor companion stamps (transient != 0). */
 int orc_topology_build_ex(const nr_netlist* nl, int32_t transient, orc_topo** out);
+/* The same with the raw Min-Fill ordering given (NR_EXPORT_ORDERING_RAW of an
+ * earlier build of the same netlist); treewidth taken from the fill. */
+int orc_topology_build_ordered(const nr_netlist* nl, int32_t transient, const int32_t* ordering,
+                               int32_t ordering_len, orc_topo** out);
 double orc_pulse_value(const double* p, double time);
 double orc_sine_value(const double* p, double time);
 /* value of a source with waveform kind NR_WAVE_* at `time` (dc: its DC value) */

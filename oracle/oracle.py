@@ -150,11 +150,20 @@ def _p32(a):
 class Topology:
     """Oracle topology: pattern, stamp map, Min-Fill ordering, fill pattern."""
 
-    def __init__(self, nl, transient=False):
+    def __init__(self, nl, transient=False, ordering=None):
+        """ordering: the raw Min-Fill ordering of this netlist from an earlier
+        build (NR_EXPORT_ORDERING_RAW), which skips the literal Min-Fill."""
         self.nl = nl
         self._c, self._keep = nl.to_c()
         h = C.c_void_p()
-        rc = lib().orc_topology_build_ex(C.byref(self._c), 1 if transient else 0, C.byref(h))
+        if ordering is None:
+            rc = lib().orc_topology_build_ex(C.byref(self._c), 1 if transient else 0, C.byref(h))
+        else:
+            o = np.ascontiguousarray(ordering, np.int32)
+            lib().orc_topology_build_ordered.argtypes = [C.POINTER(_abi.NrNetlist), C.c_int32, _i32p, C.c_int32,
+                                                         C.POINTER(C.c_void_p)]
+            rc = lib().orc_topology_build_ordered(C.byref(self._c), 1 if transient else 0, _p32(o), len(o),
+                                                  C.byref(h))
         if rc:
             raise RuntimeError(f"orc_topology_build failed: {rc}")
         self.h = h

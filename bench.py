@@ -81,17 +81,19 @@ def seeded_parity(nl, fields, seeds, leg):
     from oracle import oracle as O
     k = len(seeds)
     Vo, so = O.solve_dc_batch(nl, O.Topology(nl), {f: v[:k] for f, v in fields.items()}, k, threads=8,
-                              seeds=seeds, start_step=18)
+                              seeds=seeds, start_step=18, polish=1)
     return {"instances": k, "max_err_over_tol": err_over_tol(leg["_v_sample"], Vo),
             "iterations_equal": int((leg["_it_sample"] == so["iterations"]).sum()),
-            "reference": "FP64 oracle started from the same seed at the last GMIN step"}
+            "reference": "FP64 oracle started from the same seed at the last GMIN step, one polish "
+                         "iteration"}
 
 
-def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
+def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads, topo=None):
     """Oracle over a bounded sample of the same instances on `threads` host
-    threads; returns (baseline dict, parity dict)."""
+    threads; returns (baseline dict, parity dict).  parity["_v"] holds the
+    oracle's voltages of the sample (callers drop it before printing)."""
     from oracle import oracle as O
-    T = O.Topology(nl)
+    T = topo if topo is not None else O.Topology(nl)
     n0 = min(B, threads)
     t = time.perf_counter()
     Vo, so = O.solve_dc_batch(nl, T, {k: v[:n0] for k, v in fields.items()}, n0, threads=threads)
@@ -114,7 +116,7 @@ def cpu_baseline(nl, fields, B, V_gpu, it_gpu, seconds, threads):
     parity = {"instances": n, "max_err_over_tol": float(err.max()),
               "within_tol": int((err.max(axis=1) <= 1.0).sum()),
               "iterations_equal": int((it_gpu[:n] == so["iterations"]).sum()),
-              "tolerance": "1e-9 rel + 1e-12 V abs"}
+              "tolerance": "1e-9 rel + 1e-12 V abs", "_v": Vo}
     return base, parity
 
 
@@ -122,22 +124,22 @@ SEEDER_FEATURES, SEEDER_HIDDEN, SEEDER_LAYERS = 512, 1024, 3
 
 
 def timed_solves(bt, steps, **kw):
-    """`steps` batched DC solves with per-kernel accounting; returns the
-    converged-solve rate, ms per solve, summed kernel rows and the last
-    solve's statistics and result."""
+    """`steps` batched DC solves timed without per-kernel events (as every
+    leg is), then one more solve with a HIP event pair around every launch for
+    the kernel rows; returns the converged-solve rate, ms per solve, the
+    kernel rows and the last timed solve's statistics and result."""
     from karam_amd import _abi
-    kst, conv = {}, 0
+    conv = 0
     t0 = time.perf_counter()
     for k in range(steps):
-        r = bt.solve_dc(record_timing=2, **kw)
+        r = bt.solve_dc(**kw)
         st = bt.stats()
         conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
-        for name, x in bt.kernel_stats().items():
-            acc = kst.setdefault(name, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
-            for f in acc:
-                acc[f] += x[f]
         log(f"step {k} {kw}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {r['batched_iterations']}")
     total = time.perf_counter() - t0
+    rp = bt.solve_dc(record_timing=2, **kw)
+    kst = {name: {f: x[f] for f in ("launches", "ms", "bytes", "flops")} for name, x in bt.kernel_stats().items()}
+    log(f"profiled solve {kw}: {rp['solve_ms']:.1f} ms")
     return conv / total, total * 1e3 / steps, kst, st, r
 
 
@@ -191,13 +193,20 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, V64):
     instances drawn from a disjoint seed range (untimed, reported), then time
     `steps` solves that each predict every instance's seed on the device and
     start Newton at the last GMIN step, with the plain start as fallback, on
-    the FP64 and on the FP32 path."""
+    the FP64 and on the FP32 path.  A seeded solve takes one Newton iteration
+    after its convergence test passes (polish = 1): the test accepts KCL <
+    1e-9 A, which leaves nodes up to ~1e-6 V from the root, while the plain
+    start's last GMIN step ends at the root.  A solve counts when it is
+    KCL-converged and within the voltage tolerance (1e-9 rel + 1e-12 V) of
+    the plain FP64 answer.  Two seeds without learning start at the same step
+    for comparison: the mean of the training solutions, and the operating
+    point of the nominal netlist."""
     from karam_amd import _abi, netlists, nr
     n = T.info["n"]
     t0 = time.perf_counter()
     s = nr.Seeder(ctx, SEEDER_FEATURES, n, hidden=SEEDER_HIDDEN, n_hidden=SEEDER_LAYERS, init_seed=1)
     gen = nr.Batch(ctx, T, B)
-    done = 0
+    done, vsum, vcnt = 0, np.zeros(n), 0
     while done < a.seeder_train:
         m = min(B, a.seeder_train - done)
         ftr = netlists.mc_fields(nl, B, seed=500000 + done)
@@ -206,7 +215,10 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, V64):
         st = gen.stats()
         ok = st["status"] == _abi.NR_STATUS_CONVERGED
         ok[m:] = False
-        s.record_solution(netlists.mc_features(nl, ftr, SEEDER_FEATURES)[ok], gen.voltages()[ok])
+        Vg = gen.voltages()[ok]
+        s.record_solution(netlists.mc_features(nl, ftr, SEEDER_FEATURES)[ok], Vg)
+        vsum += Vg.sum(axis=0)
+        vcnt += len(Vg)
         done += m
     gen.close()
     t_data = time.perf_counter() - t0
@@ -218,148 +230,215 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, V64):
     seed_v = s.predict_seed(X[:8])
     dims = [SEEDER_FEATURES] + [SEEDER_HIDDEN] * SEEDER_LAYERS + [(n + 63) // 64 * 64]
     flops = 2.0 * B * sum(x * y for x, y in zip(dims[:-1], dims[1:]))
-    out = {}
-    for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
-        bt.set_seed_from(s, X)
-        bt.solve_dc(seeded_start_step=18, precision=prec)         # warm-up
-        conv, t_pred, sweeps = 0, 0.0, 0
+    mean_seed = np.tile(vsum / max(1, vcnt), (B, 1))
+    nom = nr.Batch(ctx, T, 1)
+    nom.solve_dc()
+    nominal_seed = np.tile(nom.voltages()[0], (B, 1))
+    nom.close()
+
+    def counted(st, V):
+        ok = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
+        within = (np.abs(V - V64) <= 1e-9 * np.abs(V64) + 1e-12).all(axis=1)
+        return ok & within
+
+    def leg(prec, set_seed, label):
+        set_seed()
+        bt.solve_dc(seeded_start_step=18, precision=prec, polish=1)         # warm-up
+        t_set, sweeps = 0.0, 0
         t0 = time.perf_counter()
         for k in range(a.steps):
             tp = time.perf_counter()
-            bt.set_seed_from(s, X)
-            t_pred += time.perf_counter() - tp
-            r = bt.solve_dc(seeded_start_step=18, precision=prec)
+            set_seed()
+            t_set += time.perf_counter() - tp
+            r = bt.solve_dc(seeded_start_step=18, precision=prec, polish=1)
             st = bt.stats()
-            conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
             sweeps = r["batched_iterations"]
-            log(f"seeded {key} step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
+            log(f"{label} step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
         total = time.perf_counter() - t0
+        # every step solves the same instances from the same seeds, so the last
+        # step's count is every step's
         V = bt.voltages()
-        out[key] = {
+        conv = int(counted(st, V).sum()) * a.steps
+        return {
             "value": conv / total, "unit": "solves/s", "ms_per_step": total * 1e3 / a.steps,
             "speedup_vs_plain": None, "batched_sweeps_per_step": sweeps,
             "fallback_instances": int((st["seeded"] == 0).sum()),
+            "counted_instances": int(counted(st, V).sum()),
             "iterations": iter_stats(st["iterations"]),
             "max_err_over_tol_vs_plain_fp64": err_over_tol(V, V64),
-            "predict_ms_per_step": t_pred * 1e3 / a.steps,
+            "seed_ms_per_step": t_set * 1e3 / a.steps,
             "_v_sample": V[:8], "_it_sample": st["iterations"][:8],
         }
+
+    out = {}
+    for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
+        out[key] = leg(prec, lambda: bt.set_seed_from(s, X), f"seeded {key}")
+    for key, seed in (("mean_seed_fp32_refine", mean_seed), ("nominal_seed_fp32_refine", nominal_seed)):
+        out[key] = leg(_abi.NR_PREC_FP32_REFINE, lambda sd=seed: bt.set_seed(sd), key)
+        for k in [k for k in out[key] if k.startswith("_")]:
+            del out[key][k]
     out["seeder"] = {"features": SEEDER_FEATURES, "hidden": SEEDER_HIDDEN, "layers": SEEDER_LAYERS, "outputs": n,
                      "train_instances": int(s.history()), "train_data_s": t_data, "train_s": t_train,
-                     "final_loss_normalised": loss,
-                     "predict_tflops_wall": flops / (out["fp64"]["predict_ms_per_step"] * 1e-3) / 1e12,
+                     "final_loss_normalised": loss, "polish": 1,
+                     "predict_tflops_wall": flops / (out["fp64"]["seed_ms_per_step"] * 1e-3) / 1e12,
                      "mfma_peak_tflops_bf16": 2500.0}
     out["_seed_sample"] = seed_v
     return out
 
 
+# The random-logic configuration: NAND2 / NAND3 / INV gates (fan-in 1-3,
+# fan-out <= 4), 20,000 nodes and 39,934 level-1 MOSFETs (every gate brings
+# two MOSFETs per node), about 17 gate levels of 640 gates; as
+# tests/golden/make_logic20k.py, whose fixture holds its Min-Fill ordering.
+LOGIC20K = dict(n_target=20000, seed=1, width=640, p_inv=0.3, p_nand3=0.1)
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def host_info(threads):
+    """The host the CPU baseline ran on: CPU model, logical CPUs, the CPUs
+    this process may use, and the threads the oracle used."""
+    model = None
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "threads_used": threads}
+
+
+def cpu_threads(a):
+    """Oracle threads: every CPU this process may use, at most 16 (the host
+    share of one GPU on the benchmark machines)."""
+    return a.cpu_threads or min(16, len(os.sched_getaffinity(0)))
+
+
 def logic_workload(a):
-    """Random CMOS logic (levelised inverter DAG with switching PULSE
-    inputs, level-1 MOSFETs), 2048 Monte-Carlo instances (VTO sigma 30 mV,
-    KP sigma 3 %): DC operating points, then a fixed-step backward-Euler
-    transient from them and the reference's trapezoidal transient with the
-    arbiter's step control over the same span.  The default size (2000 nodes) is the largest at
-    which the reference's Newton method converges on this family."""
+    """Random CMOS logic at the configured size (LOGIC20K: 20,000 nodes,
+    39,934 level-1 MOSFETs), 2048 Monte-Carlo instances (per-device VTO
+    sigma 30 mV, KP sigma 3 %): DC operating points on the plain FP64 path and
+    on the FP32 + FP64-refinement path, then a 500-step backward-Euler
+    transient of 0.5 ns from the operating points on both paths, with the
+    FP32 path's per-node error against FP64 and the GPU's against the oracle
+    on a bounded sample.  value = converged FP64 DC solves/s."""
     from karam_amd import _abi, netlists, nr
     B = a.instances if a.instances != 4096 else 2048
-    nl = netlists.random_logic(a.logic_nodes, seed=2, p_inv=1.0)
+    kw = dict(LOGIC20K) if a.logic_nodes == LOGIC20K["n_target"] else \
+        dict(n_target=a.logic_nodes, seed=1, width=max(32, a.logic_nodes // 31), p_inv=0.3, p_nand3=0.1)
+    nl = netlists.random_logic(**kw)
     f = netlists.mc_fields(nl, B, seed=1000, sigma_vto=0.03, sigma_kp=0.03)
     ctx = nr.Context(0)
     t = time.perf_counter()
     T = nr.Topology(nl, ctx)
+    symbolic_ms = (time.perf_counter() - t) * 1e3
     Tt = nr.Topology(nl, ctx, transient=True)
-    symbolic_ms = (time.perf_counter() - t) * 1e3 / 2
     log(f"logic n={T.info['n']} mos={T.info['n_mos']} nnz_f={T.info['nnz_f']} tw={T.info['treewidth']} "
-        f"levels={T.info['n_levels']} max_front={T.info['max_front']}")
+        f"levels={T.info['n_levels']} max_front={T.info['max_front']} symbolic {symbolic_ms:.0f} ms")
     bt = nr.Batch(ctx, T, B)
     bt.set_fields(f)
     for _ in range(a.warmup):
         bt.solve_dc()
-    conv = 0
-    t0 = time.perf_counter()
-    for k in range(a.steps):
-        r = bt.solve_dc()
-        st = bt.stats()
-        conv += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
-        log(f"dc step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {r['batched_iterations']}")
-    total = time.perf_counter() - t0
-    it = st["iterations"]
-    V = bt.voltages()
+    rate, ms, kst, st, r = timed_solves(bt, a.steps)
+    V, it = bt.voltages(), st["iterations"]
+    ok64 = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
+    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)            # warm-up
+    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE)
+    V32 = bt.voltages()
+    ok32 = (st32["status"] == _abi.NR_STATUS_CONVERGED) & (st32["kcl"] <= 1e-9)
+    both = ok64 & ok32
     bt.close()
-    tr = nr.Batch(ctx, Tt, B)
-    tr.set_fields(f)
-    tr.set_seed(V)
-    t0 = time.perf_counter()
-    rt = tr.transient(h=0.5e-9, steps=a.tran_steps, start=_abi.NR_TRAN_START_SEED,
-                      probes=[2, 3, T.info["n"] // 2, T.info["n"]], use_graph=a.tran_graph)
-    t_tran = time.perf_counter() - t0
-    stt = tr.stats()
-    # the reference's own transient: trapezoidal with the arbiter's step
-    # control over the same time span, from the same operating points
-    t_stop = a.tran_steps * 0.5e-9
-    t0 = time.perf_counter()
-    ra = tr.transient(h=0.5e-9, start=_abi.NR_TRAN_START_SEED, method=_abi.NR_INT_TRAP,
-                      control=_abi.NR_TRAN_ARBITER, t_stop=t_stop, probes=[2, 3, T.info["n"] // 2, T.info["n"]],
-                      use_graph=a.tran_graph)
-    t_arb = time.perf_counter() - t0
-    sta, tsa = tr.stats(), tr.tran_stats()
-    Wa, Ta = tr.waveform(), tr.waveform_times()
-    log(f"arbiter transient: {t_arb:.2f} s, completed {ra['n_converged']}/{B}, accepted {tsa['accepted'].sum()}, "
-        f"rejected {tsa['rejected'].sum()}, switched {(tsa['switched_at'] >= 0).sum()}")
+    probes = [2, 3, T.info["n"] // 2, T.info["n"]]
+    tran = {}
+    for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
+        tr = nr.Batch(ctx, Tt, B)
+        tr.set_fields(f)
+        tr.set_seed(V)
+        t0 = time.perf_counter()
+        rt = tr.transient(h=0.5e-9, steps=a.tran_steps, start=_abi.NR_TRAN_START_SEED, probes=probes,
+                          use_graph=a.tran_graph, precision=prec)
+        dt = time.perf_counter() - t0
+        stt = tr.stats()
+        done = stt["status"] == _abi.NR_STATUS_CONVERGED
+        tran[key] = {"wall_s": dt, "instances_completed": int(rt["n_converged"]),
+                     "instance_steps_per_s": rt["n_converged"] * a.tran_steps / dt,
+                     "newton_iterations_per_step": float(stt["iterations"][done].mean() / a.tran_steps)
+                     if done.any() else None,
+                     "batched_sweeps": rt["batched_iterations"],
+                     "_v": tr.voltages(), "_done": done, "_wave": tr.waveform()}
+        tr.close()
+        log(f"transient {key}: {dt:.1f} s, completed {rt['n_converged']}/{B}")
+    td = tran["fp64"]["_done"] & tran["fp32_refine"]["_done"]
     out = {
-        "metric": METRIC, "value": conv / total, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": total * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak",
+        "metric": METRIC, "value": rate, "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic: seeded levelised random inverter logic, per-device VTO sigma 30 mV and KP sigma 3 %",
-        "config": {"workload": f"logic{T.info['n']} Monte-Carlo DC operating point + 0.5 ns backward-Euler "
-                               f"transient",
-                   "instances": B, "nodes": T.info["n"], "mosfets": T.info["n_mos"], "nnz_f": T.info["nnz_f"],
-                   "treewidth": T.info["treewidth"], "symbolic_ms": symbolic_ms},
-        "nr_iters_per_solve": {"plain": {"mean": float(it.mean()), "p50": float(np.percentile(it, 50)),
-                                         "p99": float(np.percentile(it, 99))}, "seeded": None},
-        "transient": {"steps": a.tran_steps, "h_s": 0.5e-9, "start": "DC operating point",
-                      "instances_completed": int(rt["n_converged"]), "wall_s": t_tran,
-                      "instance_steps_per_s": rt["n_converged"] * a.tran_steps / t_tran,
-                      "newton_iterations_per_step": float(stt["iterations"].mean() / a.tran_steps),
-                      "batched_sweeps": rt["batched_iterations"]},
-        "transient_arbiter": {"t_stop_s": t_stop, "h_s": 0.5e-9, "method": "trapezoidal, switched to Gear-2 "
-                              "by the arbiter", "start": "DC operating point",
-                              "instances_completed": int(ra["n_converged"]), "wall_s": t_arb,
-                              "accepted_steps": int(tsa["accepted"].sum()),
-                              "rejected_attempts": int(tsa["rejected"].sum()),
-                              "instances_switched": int((tsa["switched_at"] >= 0).sum()),
-                              "accepted_steps_per_s": float(tsa["accepted"].sum() / t_arb),
-                              "newton_iterations_per_accepted_step":
-                                  float(sta["iterations"].sum() / max(1, tsa["accepted"].sum())),
-                              "rounds_batched_sweeps": ra["batched_iterations"]},
-        "roofline": None, "cpu_baseline": None,
+        "data": "synthetic: seeded levelised random NAND2/NAND3/INV logic, per-device VTO sigma 30 mV and KP "
+                "sigma 3 %",
+        "config": {"workload": f"logic{T.info['n']} Monte-Carlo DC operating point + {a.tran_steps}-step 0.5 ns "
+                               f"backward-Euler transient, FP64 and FP32 + FP64 refinement",
+                   "netlist": kw, "instances": B, "nodes": T.info["n"], "mosfets": T.info["n_mos"],
+                   "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
+                   "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": r["batched_iterations"]},
+        "converged_fp64": int(ok64.sum()),
+        "nr_iters_per_solve": {"plain": iter_stats(it[ok64]) if ok64.any() else None,
+                               "plain_fp32_refine": iter_stats(st32["iterations"][ok32]) if ok32.any() else None},
+        "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst),
+        "fp32_refine": {"value": rate32, "unit": "solves/s", "ms_per_step": ms32,
+                        "speedup_vs_plain": rate32 / rate if rate else None, "converged": int(ok32.sum()),
+                        "max_err_over_tol_vs_plain_fp64": err_over_tol(V32[both], V[both]) if both.any() else None,
+                        "roofline": roofline(kst32, traffic=False)},
+        "transient": {"steps": a.tran_steps, "h_s": 0.5e-9, "start": "DC operating point (FP64)",
+                      "fp64": {k: v for k, v in tran["fp64"].items() if not k.startswith("_")},
+                      "fp32_refine": {k: v for k, v in tran["fp32_refine"].items() if not k.startswith("_")},
+                      "fp32_max_err_over_tol_vs_fp64_final": err_over_tol(tran["fp32_refine"]["_v"][td],
+                                                                          tran["fp64"]["_v"][td])
+                      if td.any() else None},
+        "cpu_baseline": None,
     }
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        base, parity = cpu_baseline(nl, f, B, V, it, a.cpu_seconds, threads)
+        threads = cpu_threads(a)
+        To = oracle_topology(nl, kw, T)
+        base, parity = cpu_baseline(nl, f, B, V, it, a.cpu_seconds, threads, topo=To)
+        base["host"] = host_info(threads)
         out["cpu_baseline"] = base
         out["parity_sample"] = parity
-        # the arbiter transient of the first `threads` instances on the host
-        m = min(B, threads)
-        t0 = time.perf_counter()
-        _, Wo, so, To = O.solve_transient_batch(
-            nl, O.Topology(nl, transient=True), {k: v[:m] for k, v in f.items()}, m, h=0.5e-9, start=1, v0=V[:m],
-            method=O.INT_TRAP, control=O.TRAN_ARBITER, t_stop=t_stop, threads=threads,
-            probes=[2, 3, T.info["n"] // 2, T.info["n"]], full=True)
-        t_cpu = time.perf_counter() - t0
-        same = [bool(np.array_equal(Wa[b], Wo[b]) and np.array_equal(Ta[b], To[b]) and
-                     tsa["accepted"][b] == so["steps_done"][b] and sta["iterations"][b] == so["iterations"][b])
-                for b in range(m)]
-        out["transient_arbiter"]["cpu_baseline"] = {
-            "value": float(so["steps_done"].sum() / t_cpu), "unit": "accepted steps/s", "cores": threads,
-            "kind": "port", "sample": f"first {m} of {B} instances over the same span, FP64 oracle, one instance "
-                                      f"per OpenMP thread"}
-        out["transient_arbiter"]["parity_sample"] = {"instances": m, "bit_identical": int(sum(same))}
-        log(f"cpu baseline {base['value']:.1f} solves/s; arbiter transient {so['steps_done'].sum() / t_cpu:.1f} "
-            f"accepted steps/s on {threads} threads; bit-identical {sum(same)}/{m}")
+        # the FP32 path's per-node error against the FP64 oracle on the sample
+        k = parity["instances"]
+        Vo = parity.pop("_v")
+        sel = ok32[:k]
+        parity["fp32_refine_max_err_over_tol_vs_oracle"] = err_over_tol(V32[:k][sel], Vo[sel]) if sel.any() else None
+        # the transient's first steps of two instances against the oracle
+        m, s = min(2, B), min(a.tran_steps, a.tran_check_steps)
+        To_t = oracle_topology(nl, kw, Tt, transient=True)
+        _, Wo, so = O.solve_transient_batch(nl, To_t, {x: y[:m] for x, y in f.items()}, m, h=0.5e-9, steps=s,
+                                            start=1, v0=V[:m], probes=probes, threads=m)
+        W = tran["fp64"]["_wave"][:m, :s + 1]
+        out["transient"]["parity_sample"] = {
+            "instances": m, "steps": s, "bit_identical": int(sum(np.array_equal(W[b], Wo[b, :s + 1]) for b in range(m))),
+            "reference": "FP64 oracle (same start, same steps), probe waveforms"}
+        log(f"cpu baseline {base['value']:.2f} solves/s on {threads} threads; parity {parity}")
     print(json.dumps(out), flush=True)
     return 0
+
+
+def oracle_topology(nl, kw, T, transient=False):
+    """The oracle's topology of `nl`.  For the configured netlist its literal
+    Min-Fill ordering comes from the fixture (recomputing it takes about a
+    quarter of an hour), checked against the library's; otherwise the oracle
+    runs Min-Fill itself."""
+    from karam_amd import _abi
+    from oracle import oracle as O
+    fx = os.path.join(GOLDEN, "logic20k_ordering.npz")
+    if kw == LOGIC20K and os.path.exists(fx):
+        with np.load(fx, allow_pickle=False) as g:
+            o = g["ordering_raw"]
+        if not np.array_equal(o, T.export(_abi.NR_EXPORT_ORDERING_RAW)):
+            raise RuntimeError("the library's Min-Fill ordering differs from the oracle's fixture")
+        return O.Topology(nl, transient=transient, ordering=o)
+    return O.Topology(nl, transient=transient)
 
 
 def small_signal_batch(a):
@@ -416,7 +495,7 @@ def ac_workload(a):
     }
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(a)
         m = min(B, threads)
         To = O.Topology(nl, transient=True)
         t0 = time.perf_counter()
@@ -465,7 +544,7 @@ def noise_workload(a):
     }
     if not a.no_cpu:
         from oracle import oracle as O
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(a)
         m = min(B, threads)
         To = O.Topology(nl, transient=True)
         t0 = time.perf_counter()
@@ -530,8 +609,10 @@ def hightw_workload(a):
         "cpu_baseline": None,
     }
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(a)
         base, parity = cpu_baseline(nl, f, B, V, it, a.cpu_seconds, threads)
+        parity.pop("_v")
+        base["host"] = host_info(threads)
         out["cpu_baseline"], out["parity_sample"] = base, parity
         log(f"hightw cpu baseline {base['value']:.1f} solves/s; parity {parity}")
     print(json.dumps(out), flush=True)
@@ -670,7 +751,9 @@ def main():
     ap.add_argument("--mixed-instances", type=int, default=10000)
     ap.add_argument("--mixed-precision", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--mixed-streams", type=int, default=4, help="contexts (HIP streams) solving circuits at once")
-    ap.add_argument("--logic-nodes", type=int, default=2000)
+    ap.add_argument("--logic-nodes", type=int, default=20000)
+    ap.add_argument("--tran-check-steps", type=int, default=40,
+                    help="logic workload: transient steps of the oracle's sample")
     ap.add_argument("--hightw-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
@@ -780,21 +863,35 @@ def main():
         sd = seeded_path(a, ctx, nl, T, bt, fields, B, V)
         seeds = sd.pop("_seed_sample")
         out["seeded"] = sd
-        for key in ("fp64", "fp32_refine"):
+        for key in ("fp64", "fp32_refine", "mean_seed_fp32_refine", "nominal_seed_fp32_refine"):
             sd[key]["speedup_vs_plain"] = sd[key]["value"] / rate if rate else None
         out["nr_iters_per_solve"]["seeded"] = sd["fp64"].pop("iterations")
         out["nr_iters_per_solve"]["seeded_fp32_refine"] = sd["fp32_refine"].pop("iterations")
         out["target_speedup"] = {"seeded_fp32_refine_over_plain_fp64": sd["fp32_refine"]["speedup_vs_plain"],
                                  "goal": 2.0}
     if not a.no_cpu:
-        threads = a.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = cpu_threads(a)
         base, parity = cpu_baseline(nl, fields, B, V, it, a.cpu_seconds, threads)
+        parity.pop("_v")
+        base["host"] = host_info(threads)
         out["cpu_baseline"] = base
         out["parity_sample"] = parity
         if seeds is not None:
             for key in ("fp64", "fp32_refine"):
                 out["seeded"][key]["parity_sample"] = seeded_parity(nl, fields, seeds, out["seeded"][key])
-        log(f"cpu baseline {base['value']:.1f} solves/s on {threads} threads; parity {parity}")
+        # every path against a tight FP64 root: the oracle's plain solve and two
+        # more Newton iterations, on the first 8 instances
+        from oracle import oracle as O
+        k = min(8, B)
+        root, _ = O.solve_dc_batch(nl, O.Topology(nl), {x: y[:k] for x, y in fields.items()}, k, threads=threads,
+                                   polish=2)
+        acc = {"instances": k, "root": "FP64 oracle, plain start, two Newton iterations after convergence",
+               "plain_fp64": err_over_tol(V[:k], root), "plain_fp32_refine": err_over_tol(V32[:k], root)}
+        if "seeded" in out:
+            for key in ("fp64", "fp32_refine"):
+                acc["seeded_" + key] = err_over_tol(out["seeded"][key]["_v_sample"][:k], root)
+        out["accuracy_vs_root_over_tol"] = acc
+        log(f"cpu baseline {base['value']:.1f} solves/s on {threads} threads; parity {parity}; vs root {acc}")
     if "seeded" in out:
         for key in ("fp64", "fp32_refine"):
             for k in [k for k in out["seeded"][key] if k.startswith("_")]:

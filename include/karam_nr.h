@@ -293,6 +293,15 @@ typedef struct nr_solve_opts {
     int32_t trace_capacity;  /* > 0: keep each instance's last trace_capacity
                                 ConvergenceStep records (nr_batch_get_trace) */
     int32_t lu;              /* NR_LU_*: the linear-solve route              */
+    int32_t polish;          /* 0..8: Newton iterations taken after the last
+                                GMIN step's convergence test passes (the
+                                test accepts KCL < 1e-9 A, which leaves a
+                                node error up to ~KCL / G; one more step
+                                lands on the root).  They take the full
+                                Newton step (no damping: at the root the
+                                max-norm KCL no longer falls).  0 is the
+                                reference's solveDC; counted in the
+                                iterations                                   */
 } nr_solve_opts;
 NR_API void nr_solve_opts_default(nr_solve_opts* opts);
 

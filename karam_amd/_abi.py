@@ -111,7 +111,7 @@ class NrSolveOpts(C.Structure):
     _fields_ = [
         ("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("precision", C.c_int32),
         ("seeded_start_step", C.c_int32), ("record_timing", C.c_int32), ("use_graph", C.c_int32),
-        ("fallback_plain", C.c_int32), ("trace_capacity", C.c_int32), ("lu", C.c_int32),
+        ("fallback_plain", C.c_int32), ("trace_capacity", C.c_int32), ("lu", C.c_int32), ("polish", C.c_int32),
     ]
 
 

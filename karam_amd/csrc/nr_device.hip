@@ -155,12 +155,13 @@ struct DevTopo {
     const int *row_ptr, *col_idx;
 };
 
-// lu_fb: Newton steps of the instance that the dense LU solved (NR_LU_AUTO)
+// lu_fb: Newton steps of the instance that the dense LU solved (NR_LU_AUTO);
+// polished: Newton iterations taken after the last GMIN step converged
 struct InstState {
     int step, it, total, ref;
     int status, done, singular, seeded;
     double prev_kcl, kcl;
-    int lu_fb, pad_;
+    int lu_fb, polished;
 };
 
 // Per-instance transient step control: the instance's time, the step and
@@ -249,6 +250,9 @@ struct DevBatch {
     // dn_slots workspaces of n * n doubles (null when the route does not use
     // the dense LU)
     int lu;
+    // DC: Newton iterations to take after the last GMIN step converges
+    // (nr_solve_opts.polish)
+    int polish;
     int dn_slots;
     unsigned long long* dn_acc;
     int* dn_bad;
@@ -695,7 +699,7 @@ __global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P) {
     if (threadIdx.x == 0) {
         st.step = 0; st.it = 0;
         st.status = NR_STATUS_NOT_CONVERGED; st.done = 0; st.singular = 0; st.seeded = 0;
-        st.prev_kcl = 0.0; st.kcl = 0.0;
+        st.prev_kcl = 0.0; st.kcl = 0.0; st.polished = 0;
         P.st[b] = st;
     }
 }
@@ -768,7 +772,7 @@ __global__ __launch_bounds__(256) void k_init(DevTopo T, DevBatch P, int seeded_
         s.status = NR_STATUS_NOT_CONVERGED; s.done = mode != 0 ? DONE_ARMED : DONE_RUNNING; s.singular = 0;
         s.seeded = use_seed;
         s.prev_kcl = 0.0; s.kcl = 0.0;
-        s.lu_fb = 0; s.pad_ = 0;
+        s.lu_fb = 0; s.polished = 0;
         P.st[b] = s;
         if (mode == 0) {
             DiagState d;
@@ -2266,7 +2270,7 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense(DevTopo T, DevBatch P, 
 // last three residuals rise
 __device__ void diag_record(const DevBatch& P, int b, const InstState& st, double kcl, double dv, double prev) {
     DiagState d = P.diag[b];
-    const double damping = (st.it > 0 && kcl >= prev) ? 0.5 : 1.0;
+    const double damping = (st.it > 0 && kcl >= prev && st.polished == 0) ? 0.5 : 1.0;
     if (P.trace_cap > 0) {
         ConvRec r;
         r.residual = kcl; r.max_dv = dv; r.damping = damping;
@@ -2341,7 +2345,10 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     }
     double kcl = block_kcl(T, P, b, vn, red, res);
     const double prev_final = st.prev_kcl;
-    if (!tran && st.it > 0 && kcl >= st.prev_kcl) {
+    // a polish iteration (after the last GMIN step converged) takes the full
+    // Newton step: at the root the max-norm KCL has reached its gmin floor and
+    // no longer falls, while the voltages still move
+    if (!tran && st.it > 0 && kcl >= st.prev_kcl && st.polished == 0) {
         for (int pass = 0; pass < 2; ++pass) {
             for (int i = threadIdx.x; i < n; i += blockDim.x) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
             __syncthreads();
@@ -2387,7 +2394,11 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         }
         P.st[b] = st;
     } else if (threadIdx.x == 0) {
-        if (conv) {
+        if (conv && st.step == nsteps - 1 && st.polished < P.polish && st.it + 1 < max_iter) {
+            // one more Newton iteration from the converged iterate
+            st.polished++;
+            st.it++;
+        } else if (conv) {
             st.total += st.it + 1;
             st.ref += st.it + 1;
             if (st.step == nsteps - 1) { st.status = NR_STATUS_CONVERGED; st.done = 1; }

@@ -235,16 +235,21 @@ def rd_mesh(side: int = 64, seed: int = 1, n_isrc: int = 8,
 
 
 def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: float = 0.05,
-              sigma_vto: float = 0.0, sigma_kp: float = 0.0) -> dict:
+              sigma_vto: float = 0.0, sigma_kp: float = 0.0, sigma_isrc: float = 0.0,
+              sigma_vsrc: float = 0.0) -> dict:
     """Per-instance Monte-Carlo parameter arrays [B, count] (netlist order):
     R and IS multiplied by max(0.05, 1 + sigma z); VTO += sigma_vto z,
-    KP *= max(0.05, 1 + sigma_kp z); z ~ N(0, 1), instance b drawn from
-    seed + b so any instance can be regenerated on its own."""
+    KP *= max(0.05, 1 + sigma_kp z); current and voltage source values (a
+    corner's supply) multiplied by max(0.05, 1 + sigma z) when their sigma is
+    set; z ~ N(0, 1), instance b drawn from seed + b so any instance can be
+    regenerated on its own (the source draws come last, so they leave the
+    other draws of an instance unchanged)."""
     out = {}
     nr, nd, nm = len(nl.res_ohms), len(nl.dio_is), len(nl.mos_w)
     R = np.empty((B, nr)); Isa = np.empty((B, nd))
     kp0, vto0 = nl.card_params()
     KP = np.empty((B, nm)); VT = np.empty((B, nm))
+    ISRC = np.empty((B, len(nl.isrc_amps))); VSRC = np.empty((B, len(nl.vsrc_volts)))
     for b in range(B):
         rng = np.random.default_rng(seed + b)
         R[b] = nl.res_ohms * np.maximum(0.05, 1.0 + sigma_r * rng.standard_normal(nr))
@@ -252,6 +257,10 @@ def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: f
         if nm:
             VT[b] = vto0 + sigma_vto * rng.standard_normal(nm)
             KP[b] = kp0 * np.maximum(0.05, 1.0 + sigma_kp * rng.standard_normal(nm))
+        if sigma_isrc:
+            ISRC[b] = nl.isrc_amps * np.maximum(0.05, 1.0 + sigma_isrc * rng.standard_normal(len(nl.isrc_amps)))
+        if sigma_vsrc:
+            VSRC[b] = nl.vsrc_volts * np.maximum(0.05, 1.0 + sigma_vsrc * rng.standard_normal(len(nl.vsrc_volts)))
     if nr:
         out[_abi.NR_FIELD_RES_OHMS] = R
     if nd:
@@ -259,14 +268,21 @@ def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: f
     if nm and (sigma_vto or sigma_kp):
         out[_abi.NR_FIELD_MOS_KP] = KP
         out[_abi.NR_FIELD_MOS_VTO] = VT
+    if sigma_isrc and len(nl.isrc_amps):
+        out[_abi.NR_FIELD_ISRC_AMPS] = ISRC
+    if sigma_vsrc and len(nl.vsrc_volts):
+        out[_abi.NR_FIELD_VSRC_VOLTS] = VSRC
     return out
 
 
 def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
     """Seeder input features of Monte-Carlo instances, [B, n_features] FP32:
-    the per-device deviations from the template netlist (log ratios for R,
-    IS, KP, capacitance and source values; differences for VTO), in netlist
-    order and field order, averaged over n_features contiguous blocks."""
+    deviations from the template netlist (log ratios for R, IS, KP,
+    capacitance and current-source values; differences for VTO and
+    voltage-source values).  The independent sources' deviations come first,
+    one feature each (a supply or an injected current moves every node);
+    the devices' deviations, in netlist order and field order, are averaged
+    over the remaining features in contiguous blocks."""
     kp0, vto0 = nl.card_params()
     nominal = {
         _abi.NR_FIELD_RES_OHMS: (nl.res_ohms, "log"), _abi.NR_FIELD_DIO_IS: (nl.dio_is, "log"),
@@ -274,7 +290,7 @@ def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
         _abi.NR_FIELD_ISRC_AMPS: (nl.isrc_amps, "log"), _abi.NR_FIELD_VSRC_VOLTS: (nl.vsrc_volts, "diff"),
         _abi.NR_FIELD_CAP_FARADS: (nl.cap_farads, "log"),
     }
-    cols = []
+    src, dev = [], []
     for fid in sorted(fields):
         x = np.asarray(fields[fid], dtype=np.float64)
         base, kind = nominal[fid]
@@ -284,15 +300,21 @@ def mc_features(nl: Netlist, fields: dict, n_features: int) -> np.ndarray:
             d[~np.isfinite(d)] = 0.0
         else:
             d = x - base
-        cols.append(d)
-    if not cols:
+        (src if fid in (_abi.NR_FIELD_ISRC_AMPS, _abi.NR_FIELD_VSRC_VOLTS) else dev).append(d)
+    if not src and not dev:
         raise ValueError("no perturbed fields")
-    D = np.concatenate(cols, axis=1)
-    blocks = np.array_split(np.arange(D.shape[1]), n_features)
-    out = np.zeros((D.shape[0], n_features), dtype=np.float32)
-    for k, idx in enumerate(blocks):
-        if len(idx):
-            out[:, k] = D[:, idx].mean(axis=1)
+    B = (src or dev)[0].shape[0]
+    S = np.concatenate(src, axis=1) if src else np.zeros((B, 0))
+    if S.shape[1] > n_features:
+        raise ValueError("more perturbed sources than features")
+    out = np.zeros((B, n_features), dtype=np.float32)
+    out[:, :S.shape[1]] = S
+    if dev:
+        D = np.concatenate(dev, axis=1)
+        nb = n_features - S.shape[1]
+        for k, idx in enumerate(np.array_split(np.arange(D.shape[1]), nb)):
+            if len(idx):
+                out[:, S.shape[1] + k] = D[:, idx].mean(axis=1)
     return out
 
 

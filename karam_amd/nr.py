@@ -284,12 +284,12 @@ class Topology:
 
 
 def solve_opts(max_iter=50, gmin_steps=19, precision=_abi.NR_PREC_FP64, seeded_start_step=18,
-               record_timing=0, use_graph=0, fallback_plain=1, trace_capacity=0, lu=_abi.NR_LU_AUTO):
+               record_timing=0, use_graph=0, fallback_plain=1, trace_capacity=0, lu=_abi.NR_LU_AUTO, polish=0):
     o = _abi.NrSolveOpts()
     lib().nr_solve_opts_default(C.byref(o))
     o.max_iter, o.gmin_steps, o.precision = max_iter, gmin_steps, precision
     o.seeded_start_step, o.record_timing, o.use_graph = seeded_start_step, record_timing, use_graph
-    o.fallback_plain, o.trace_capacity, o.lu = fallback_plain, trace_capacity, int(lu)
+    o.fallback_plain, o.trace_capacity, o.lu, o.polish = fallback_plain, trace_capacity, int(lu), int(polish)
     return o
 
 

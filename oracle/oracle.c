@@ -1531,7 +1531,7 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 void orc_dc_opts_default(orc_dc_opts* o) {
     o->max_iter = 50; o->gmin_steps = 19; o->lu_kind = ORC_LU_AUTO;
     o->start_step = 0; o->use_seed = 0; o->precision = ORC_PREC_FP64;
-    o->trace_cap = 0; o->pad = 0; o->trace = NULL;
+    o->trace_cap = 0; o->polish = 0; o->trace = NULL;
 }
 
 /* The NR convergence diagnostics of solveDC (solvers/circuitsim_solver.cpp:
@@ -1611,7 +1611,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     if (o.use_seed) memcpy(v, v_io, sizeof(double) * (size_t)n);
     else orc_seed(nl, v);
     int converged = 0, singular = 0;
-    int32_t total = 0, ref_total = 0, steps_done = 0;
+    int32_t total = 0, ref_total = 0, steps_done = 0, polished = 0;
     double final_kcl = 0.0, last_gmin = 1.0;
     dc_diag dg;
     memset(&dg, 0, sizeof(dg));
@@ -1641,7 +1641,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
             for (int32_t i = 0; i < n; ++i) if (isnan(vn[i]) || isinf(vn[i])) vn[i] = 0.0;
             apply_pn_limits_soa(nl, s, vn, v);
             double kcl = kcl_soa(nl, s, vn, res, NULL);
-            if (it > 0 && kcl >= final_kcl) {
+            if (it > 0 && kcl >= final_kcl && polished == 0) {   /* polish: full Newton steps */
                 for (int32_t i = 0; i < n; ++i) vn[i] = v[i] + 0.5 * (vn[i] - v[i]);
                 kcl = kcl_soa(nl, s, vn, res, NULL);
                 if (kcl >= final_kcl) {
@@ -1659,7 +1659,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
                     const double d = fabs(vn[i] - v[i]);
                     if (d > max_dv) max_dv = d;
                 }
-                const double damping = (it > 0 && kcl >= final_kcl) ? 0.5 : 1.0;
+                const double damping = (it > 0 && kcl >= final_kcl && polished == 0) ? 0.5 : 1.0;
                 if (o.trace_cap > 0 && o.trace) {
                     orc_conv_step* r = &o.trace[dg.n % o.trace_cap];
                     r->residual = kcl; r->max_dv = max_dv; r->damping = damping;
@@ -1671,6 +1671,10 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
             double tol = (k == o.gmin_steps - 1) ? 1e-12 : 1.0;
             if (orc_check_convergence(n, vn, v, kcl, tol)) {
                 memcpy(v, vn, sizeof(double) * (size_t)n);
+                if (k == o.gmin_steps - 1 && polished < o.polish && it + 1 < o.max_iter) {
+                    ++polished;      /* one more Newton iteration from the converged iterate */
+                    continue;
+                }
                 gconv = 1;
                 break;
             }

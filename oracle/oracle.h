@@ -131,7 +131,8 @@ typedef struct orc_dc_opts {
     int32_t use_seed;     /* 1: v_inout holds the start vector */
     int32_t precision;    /* ORC_PREC_* */
     int32_t trace_cap;    /* > 0: ring of the last trace_cap records */
-    int32_t pad;
+    int32_t polish;       /* Newton iterations after the last GMIN step
+                             converges (the library's nr_solve_opts.polish) */
     orc_conv_step* trace; /* [trace_cap] (per instance in the batch call) */
 } orc_dc_opts;
 typedef struct orc_dc_out {

@@ -28,7 +28,7 @@ class OrcConvStep(C.Structure):
 class OrcDcOpts(C.Structure):
     _fields_ = [("max_iter", C.c_int32), ("gmin_steps", C.c_int32), ("lu_kind", C.c_int32),
                 ("start_step", C.c_int32), ("use_seed", C.c_int32), ("precision", C.c_int32),
-                ("trace_cap", C.c_int32), ("pad", C.c_int32), ("trace", C.POINTER(OrcConvStep))]
+                ("trace_cap", C.c_int32), ("polish", C.c_int32), ("trace", C.POINTER(OrcConvStep))]
 
 
 class OrcTranOpts(C.Structure):
@@ -188,8 +188,8 @@ class Topology:
 
 
 def dc_opts(max_iter=50, gmin_steps=19, lu=LU_AUTO, start_step=0, use_seed=0, precision=PREC_FP64,
-            trace_cap=0, trace=None):
-    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision, trace_cap, 0, trace)
+            trace_cap=0, trace=None, polish=0):
+    return OrcDcOpts(max_iter, gmin_steps, lu, start_step, use_seed, precision, trace_cap, polish, trace)
 
 
 def classify_history(residual, max_dv, singular=False):
@@ -218,7 +218,8 @@ def solve_dc(nl, topo: Topology | None = None, lu=LU_AUTO, max_iter=50, gmin_ste
 
 
 def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_AUTO, threads=0,
-                   max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0, trace_cap=0):
+                   max_iter=50, gmin_steps=19, precision=PREC_FP64, seeds=None, start_step=0, trace_cap=0,
+                   polish=0):
     """Solve B Monte-Carlo instances; fields[fid] is a [B, count] array;
     seeds (optional, [B, n]) start every instance at GMIN step start_step.
     trace_cap > 0 adds stats["trace"]: the last trace_cap ConvergenceStep
@@ -238,7 +239,7 @@ def solve_dc_batch(nl, topo: Topology, fields: dict, B: int, lu=LU_AUTO, threads
     outs = (OrcDcOut * B)()
     tr = (OrcConvStep * (B * trace_cap))() if trace_cap > 0 else None
     o = dc_opts(max_iter, gmin_steps, lu, start_step if seeds is not None else 0, 0 if seeds is None else 1,
-                precision, trace_cap, C.cast(tr, C.POINTER(OrcConvStep)) if tr is not None else None)
+                precision, trace_cap, C.cast(tr, C.POINTER(OrcConvStep)) if tr is not None else None, polish)
     lib().orc_solve_dc_batch(C.byref(c), topo.h, B, arrs, C.byref(o), threads, _p64(v), outs)
     stats = {
         "iterations": np.array([x.iterations for x in outs]),

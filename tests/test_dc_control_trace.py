@@ -191,12 +191,14 @@ def converged(vn, vo, k, tol):
     return not (k > tol)
 
 
-def solve_dc(c, max_iter=50, gmin_steps=19, mutant=None):
+def solve_dc(c, max_iter=50, gmin_steps=19, mutant=None, polish=0):
     """Returns (records [(residual, max_dv, damping, step, iteration)],
-    iterations, reference-accounting iterations, status)."""
+    iterations, reference-accounting iterations, status, voltages).  polish:
+    Newton iterations taken after the last GMIN step converges (the library's
+    option; 0 is the reference)."""
     ge = (lambda x, y: x > y) if mutant == "gt" else (lambda x, y: x >= y)
     v = seed(c)
-    rec, total, ref, final, status = [], 0, 0, 0.0, 1
+    rec, total, ref, final, status, polished = [], 0, 0, 0.0, 1, 0
     for k in range(gmin_steps):
         gmin = max(10.0 ** -(3.0 + k * 0.5), 1e-12)
         conv = singular = False
@@ -213,16 +215,20 @@ def solve_dc(c, max_iter=50, gmin_steps=19, mutant=None):
             vn[~np.isfinite(vn)] = 0.0
             limit(c, vn, v, sequential=mutant != "parallel")
             r = kcl(c, vn)
-            if it > 0 and ge(r, final):
+            if it > 0 and ge(r, final) and polished == 0:     # polish iterations take the full step
                 vn = v + 0.5 * (vn - v)
                 r = kcl(c, vn)
                 if mutant != "single" and ge(r, final):
                     vn = v + 0.5 * (vn - v)
                     r = kcl(c, vn)
-            rec.append((r, float(np.abs(vn - v).max()), 0.5 if (it > 0 and ge(r, final)) else 1.0, k, it))
+            rec.append((r, float(np.abs(vn - v).max()), 0.5 if (it > 0 and ge(r, final) and polished == 0) else 1.0,
+                        k, it))
             final = r
             if converged(vn, v, r, 1e-12 if k == gmin_steps - 1 else 1.0):
                 v = vn
+                if k == gmin_steps - 1 and polished < polish and it + 1 < max_iter:
+                    polished += 1
+                    continue
                 conv = True
                 total += it + 1
                 ref += it + 1
@@ -264,16 +270,18 @@ CIRCUITS = {
                               diodes=[(2, 3, IS), (3, 0, IS), (0, 4, IS)]), {"max_iter": 3}),
     # a current source driving a diode from a mid-rail seed, then pulled back
     "isrc_diode": (Circ(2, res=[(1, 2, 50.0), (2, 0, 1e5)], isrc=[(0, 1, 0.05)], diodes=[(1, 0, IS)]), {}),
+    # two extra Newton iterations after the last GMIN step converges
+    "diode_r_polish": (Circ(2, res=[(1, 2, 1e3)], vsrc=[(1, 0, 5.0)], diodes=[(2, 0, IS)]), {"polish": 2}),
     # a resistor with no value: a singular first system
     "singular": (Circ(2, res=[(1, 2, float("nan")), (2, 0, 2e3)], vsrc=[(1, 0, 5.0)]), {}),
 }
 
 
-def _oracle(c, max_iter=50):
+def _oracle(c, max_iter=50, polish=0):
     from oracle import oracle as O
     nl = c.netlist()
     T = O.Topology(nl)
-    V, s = O.solve_dc_batch(nl, T, {}, 1, max_iter=max_iter, trace_cap=4096)
+    V, s = O.solve_dc_batch(nl, T, {}, 1, max_iter=max_iter, trace_cap=4096, polish=polish)
     tr = {k: x[0] for k, x in s["trace"].items()}
     h = int(s["history"][0])
     recs = [(tr["residual"][j], tr["max_dv"][j], tr["damping"][j], tr["gmin_step"][j], tr["iteration"][j])
@@ -330,6 +338,8 @@ def test_the_traces_exercise_every_rule(runs):
     assert runs["isrc_break"][1][1:4] == (1, 2, 1)
     # singular at the first iteration: counted once, and twice by the reference
     assert runs["singular"][1][1:4] == (1, 2, 2)
+    # polish: two more iterations, counted
+    assert runs["diode_r_polish"][1][1] == runs["diode_r"][1][1] + 2
 
 
 def test_damping_doubling_is_exercised(oracle_lib):

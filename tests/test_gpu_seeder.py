@@ -53,16 +53,31 @@ def test_loss_and_gradient_match_fp64_restatement(ctx):
             o += size
 
 
-def test_trained_seeder_cuts_newton_iterations(ctx):
+FAMILIES = {
+    # the configured Monte-Carlo spread: the operating points move little, so a
+    # non-learned seed at the last GMIN step is already close
+    "rd_is_5pct": dict(),
+    # the current sources spread by 40 %: every node moves with them
+    "sources_40pct": dict(sigma_isrc=0.4),
+}
+
+
+@pytest.mark.parametrize("family", list(FAMILIES))
+def test_trained_seeder_cuts_newton_iterations(ctx, family):
     """16x16 R-diode mesh: train on 4096 solved instances, then solve 512
-    unseen instances seeded at the last GMIN step."""
+    unseen instances seeded at the last GMIN step with one polish iteration.
+    Seeded answers lie within the voltage tolerance (1e-9 rel + 1e-12 V) of
+    the plain FP64 answers.  Against the seed without learning at the same
+    step (the mean of the training solutions): never more iterations, and
+    on the family whose sources spread, clearly fewer."""
     from karam_amd import nr
     from oracle import oracle as O
+    kw = FAMILIES[family]
     nl = netlists.rd_mesh(16, seed=2)
     n, F = nl.num_nodes, 64
     T = nr.Topology(nl, ctx)
     Ntr, Nte = 4096, 512
-    ftr = netlists.mc_fields(nl, Ntr, seed=700000)
+    ftr = netlists.mc_fields(nl, Ntr, seed=700000, **kw)
     bt = nr.Batch(ctx, T, Ntr)
     bt.set_fields(ftr)
     assert bt.solve_dc()["n_converged"] == Ntr
@@ -72,31 +87,45 @@ def test_trained_seeder_cuts_newton_iterations(ctx):
     loss0 = s.train(epochs=1, batch=256, lr=0.0)       # the untrained network's loss
     loss = s.train(epochs=40, batch=256, lr=1e-3)
     assert loss < 0.7 * loss0
-    fte = netlists.mc_fields(nl, Nte, seed=900000)
+    fte = netlists.mc_fields(nl, Nte, seed=900000, **kw)
     xte = netlists.mc_features(nl, fte, F)
     te = nr.Batch(ctx, T, Nte)
     te.set_fields(fte)
     plain = te.solve_dc()
     it_plain = te.stats()["iterations"]
+    V_plain = te.voltages()
+    te.set_seed(np.tile(bt.voltages().mean(axis=0), (Nte, 1)))
+    te.solve_dc(seeded_start_step=18, polish=1)
+    st_mean = te.stats()
     te.set_seed_from(s, xte)
-    seeded = te.solve_dc(seeded_start_step=18)
+    seeded = te.solve_dc(seeded_start_step=18, polish=1)
     st = te.stats()
+    V = te.voltages()
     assert plain["n_converged"] == Nte and seeded["n_converged"] == Nte
-    assert st["kcl"].max() <= 1e-9
-    print(f"seeder loss {loss0:.3f} -> {loss:.3f}; iterations plain {it_plain.mean():.2f} "
-          f"seeded {st['iterations'].mean():.2f} (max {st['iterations'].max()})")
-    assert st["iterations"].mean() <= 6 and st["iterations"].mean() * 10 <= it_plain.mean()
+    assert st["kcl"].max() <= 1e-9 and (st["seeded"] == 1).all()
+    m_net, m_mean = st["iterations"].mean(), st_mean["iterations"].mean()
+    print(f"{family}: seeder loss {loss0:.3f} -> {loss:.3f}; iterations plain {it_plain.mean():.2f}, "
+          f"learned seed {m_net:.2f} (max {st['iterations'].max()}), training-mean seed {m_mean:.2f}")
+    assert m_net <= 7 and m_net * 10 <= it_plain.mean()
+    assert m_net <= m_mean
+    if kw:
+        assert m_net <= m_mean - 0.5
+    # within the voltage tolerance of the plain answers: two Newton paths to
+    # the same root part at the FP64 floor of this mesh (the KCL settles near
+    # 1e-11 A, about half the tolerance), so a few instances may sit just
+    # above it
+    e = (np.abs(V - V_plain) / (1e-9 * np.abs(V_plain) + 1e-12)).max(axis=1)
+    print(f"{family}: seeded vs plain FP64: max {e.max():.3f} of the tolerance, {(e <= 1).mean():.4f} within")
+    assert (e <= 1.0).mean() >= 0.99 and e.max() <= 3.0
     # the device-side seed equals the host prediction; the oracle started from
     # that seed follows the same iterations to the same voltages
     seed_v = s.predict_seed(xte)
-    V = te.voltages()
-    for b in (0, 77, 311):
-        inst = netlists.Netlist(**{**nl.__dict__})
-        inst.res_ohms = fte[_abi.NR_FIELD_RES_OHMS][b]
-        inst.dio_is = fte[_abi.NR_FIELD_DIO_IS][b]
-        vo, out = O.solve_dc(inst, O.Topology(nl), seed_v=seed_v[b], start_step=18)
-        assert out.iterations == st["iterations"][b]
-        assert np.all(np.abs(V[b] - vo) <= 1e-9 * np.abs(vo) + 1e-12)
+    pick = [0, 77, 311]
+    sub = {k: v[pick] for k, v in fte.items()}
+    Vo, so = O.solve_dc_batch(nl, O.Topology(nl), sub, 3, seeds=seed_v[pick], start_step=18, polish=1)
+    for j, b in enumerate(pick):
+        assert so["iterations"][j] == st["iterations"][b]
+        assert np.all(np.abs(V[b] - Vo[j]) <= 1e-9 * np.abs(Vo[j]) + 1e-12)
 
 
 def test_plain_fallback_after_seeded_failure(ctx):

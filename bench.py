@@ -387,6 +387,7 @@ def logic_workload(a):
         "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst),
         "fp32_refine": {"value": rate32, "unit": "solves/s", "ms_per_step": ms32,
                         "speedup_vs_plain": rate32 / rate if rate else None, "converged": int(ok32.sum()),
+                        "finished_on_fp64": int((st32["lu_fallbacks"] > 0).sum()),
                         "max_err_over_tol_vs_plain_fp64": err_over_tol(V32[both], V[both]) if both.any() else None,
                         "roofline": roofline(kst32, traffic=False)},
         "transient": {"steps": a.tran_steps, "h_s": 0.5e-9, "start": "DC operating point (FP64)",
@@ -410,6 +411,15 @@ def logic_workload(a):
         Vo = parity.pop("_v")
         sel = ok32[:k]
         parity["fp32_refine_max_err_over_tol_vs_oracle"] = err_over_tol(V32[:k][sel], Vo[sel]) if sel.any() else None
+        # where the FP32 answer differs from FP64, the difference as a current
+        # at each node (G_ii |dv_i|, G = the DC Jacobian's diagonal at the FP64
+        # answer) against the KCL acceptance of 1e-9 A: NAND stack nodes held
+        # by off transistors and gmin (G ~ 3e-12 S) take volts of freedom
+        bb = np.flatnonzero(both)[:8]
+        out["fp32_refine"]["max_node_current_diff_A_vs_plain_fp64"] = max(
+            float((O.node_conductance(netlists.instance(nl, f, b), To, V[b]) * np.abs(V32[b] - V[b])).max())
+            for b in bb) if len(bb) else None
+        out["fp32_refine"]["node_current_diff_instances"] = len(bb)
         # the transient's first steps of two instances against the oracle
         m, s = min(2, B), min(a.tran_steps, a.tran_check_steps)
         To_t = oracle_topology(nl, kw, Tt, transient=True)

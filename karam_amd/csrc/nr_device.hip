@@ -197,8 +197,9 @@ struct FinalDiag {
 
 // InstState.done in a transient: 0 solving, 1 finished, 2 the attempt's
 // Newton solve converged, 3 it failed (the step control decides), 4 the
-// next attempt is armed (k_tran_begin sets it up)
-enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3, DONE_ARMED = 4 };
+// next attempt is armed (k_tran_begin sets it up).  DC on the FP32 path: 5
+// the instance leaves the FP32 sweeps to finish on the FP64 path
+enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3, DONE_ARMED = 4, DONE_PROMOTE = 5 };
 
 struct DevBatch {
     int B, Bp;
@@ -250,6 +251,11 @@ struct DevBatch {
     // dn_slots workspaces of n * n doubles (null when the route does not use
     // the dense LU)
     int lu;
+    // DC on the FP32 path above NR_DENSE_MAX_N nodes with the AUTO route:
+    // an instance whose float solve k_lu_flag rejects (or whose float LU met
+    // a vanishing pivot) drops that iteration and finishes the solve on the
+    // FP64 path from the same iterate
+    int promote;
     // DC: Newton iterations to take after the last GMIN step converges
     // (nr_solve_opts.polish)
     int polish;
@@ -1988,7 +1994,12 @@ __global__ __launch_bounds__(256) void k_v_to_vI(DevTopo T, DevBatch P, const in
 // circuitsim_solver.cpp:291-295); NR_LU_DENSE uses that LU for every solve.
 constexpr int LINRES_ROWS = 16;          // rows per wavefront of k_linres
 constexpr double LU_ETA_FP64 = 1e-8;     // bound of the FP64 path's ratio
-constexpr double LU_ETA_FP32 = 1e-3;     // bound of the FP32 path's float system
+// The FP32 path's float system: its Newton step is an inexact Newton step
+// (the FP64 residual corrects it), which needs the float solve to cut the
+// linear residual, not to match it to rounding.  Accurate float solves of a
+// 20,000-node logic circuit near its root sit at 1e-3..2e-2 (max|b| is small
+// there); solves the missing pivoting spoils sit at 1e16 and above.
+constexpr double LU_ETA_FP32 = 0.1;
 constexpr int DENSE_THREADS = 1024;
 
 // NR_LU_AUTO, after the back substitution: per instance, max_i |b_i - (A x)_i|
@@ -2050,7 +2061,9 @@ __global__ __launch_bounds__(256) void k_linres(DevTopo T, DevBatch P, const int
 // NR_LU_AUTO: list the active instances whose solve the dense LU redoes (a
 // value that is not finite, a residual above bound * max|b|, or a pivot the
 // multifrontal LU found below 1e-25), clear the maxima for the next sweep and
-// rewind the queue.  One block.
+// rewind the queue.  With P.promote such instances are marked singular
+// instead (k_update sends them to the FP64 path) and the list stays empty.
+// One block.
 __global__ __launch_bounds__(1024) void k_lu_flag(DevBatch P, const int* __restrict__ act, double bound) {
     __shared__ int cnt;
     const int nact = *P.nact;
@@ -2064,7 +2077,10 @@ __global__ __launch_bounds__(1024) void k_lu_flag(DevBatch P, const int* __restr
         P.dn_acc[2 * (size_t)b] = 0ull;
         P.dn_acc[2 * (size_t)b + 1] = 0ull;
         P.dn_bad[b] = 0;
-        if (bad || !(mr <= bound * mb) || P.st[b].singular) P.dn_list[atomicAdd(&cnt, 1)] = b;
+        if (bad || !(mr <= bound * mb) || P.st[b].singular) {
+            if (P.promote) P.st[b].singular = 1;
+            else P.dn_list[atomicAdd(&cnt, 1)] = b;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -2299,6 +2315,17 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     const int b = act[blockIdx.x];
     const int n = T.n;
     InstState st = P.st[b];
+    if (st.singular && P.promote) {
+        // the float solve was rejected: the iterate stays, and the iteration
+        // is taken again on the FP64 path
+        if (threadIdx.x == 0) {
+            st.singular = 0;
+            st.lu_fb++;
+            st.done = DONE_PROMOTE;
+            P.st[b] = st;
+        }
+        return;
+    }
     if (st.singular) {
         // singular Jacobian: abort the GMIN step and the solve (the reference's count)
         if (threadIdx.x == 0) {
@@ -2419,6 +2446,17 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
         }
         P.st[b] = st;
     }
+}
+
+// The instances that left the FP32 sweeps resume on the FP64 path (one
+// block per instance; P is the FP64 view): the interleaved rhs goes back to
+// the linear rhs (the FP32 path keeps the KCL vector there)
+__global__ __launch_bounds__(256) void k_promote(DevTopo T, DevBatch P) {
+    const int b = blockIdx.x;
+    if (P.st[b].done != DONE_PROMOTE) return;
+    for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.rhslin[IX(i, b, P.Bp)] = rhs_lin_node(T, P, b, i);
+    __syncthreads();
+    if (threadIdx.x == 0) P.st[b].done = DONE_RUNNING;
 }
 
 // rebuild the active list in increasing instance order (single block)

@@ -15,6 +15,7 @@ Families:
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -272,6 +273,18 @@ def mc_fields(nl: Netlist, B: int, seed: int, sigma_r: float = 0.05, sigma_is: f
         out[_abi.NR_FIELD_ISRC_AMPS] = ISRC
     if sigma_vsrc and len(nl.vsrc_volts):
         out[_abi.NR_FIELD_VSRC_VOLTS] = VSRC
+    return out
+
+
+def instance(nl: Netlist, fields: dict, b: int) -> Netlist:
+    """A copy of `nl` carrying instance b's values of `fields` (the netlist
+    an instance of the batch solves)."""
+    out = dataclasses.replace(nl)
+    names = {_abi.NR_FIELD_RES_OHMS: "res_ohms", _abi.NR_FIELD_DIO_IS: "dio_is", _abi.NR_FIELD_MOS_KP: "mos_kp",
+             _abi.NR_FIELD_MOS_VTO: "mos_vto", _abi.NR_FIELD_ISRC_AMPS: "isrc_amps",
+             _abi.NR_FIELD_VSRC_VOLTS: "vsrc_volts", _abi.NR_FIELD_CAP_FARADS: "cap_farads"}
+    for fid, arr in fields.items():
+        setattr(out, names[fid], np.array(arr[b], dtype=np.float64))
     return out
 
 

@@ -1293,10 +1293,13 @@ int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, d
 /* One Newton step of the FP32 path at v: Jacobian and LU in float, rhs =
  * the FP64 Newton residual rounded to float, vn = v + the float correction
  * (FP64 update).  The Newton residual b - A v of the reference's Norton
- * system is the KCL audit vector minus the gmin diagonal current. */
+ * system is the KCL audit vector minus the gmin diagonal current.
+ * With `promote` (DC, AUTO route, n > NR_DENSE_MAX_N) a float solve the
+ * residual check rejects sets *fallback = 2 and leaves vn unset: the caller
+ * takes the iteration again on the FP64 path and stays there. */
 static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_topo* t, const double* v,
-                           double gmin, const tran_ctx* tc, int32_t lu_kind, double* vals, double* lin, float* a32,
-                           float* r32, float* x32, double* res, double* vn, int32_t* fallback) {
+                           double gmin, const tran_ctx* tc, int32_t lu_kind, int promote, double* vals, double* lin,
+                           float* a32, float* r32, float* x32, double* res, double* vn, int32_t* fallback) {
     const int32_t n = nl->num_nodes;
     *fallback = 0;
     int rc = assemble_f32(nl, s, t, v, gmin, tc, vals, lin, a32);
@@ -1307,7 +1310,7 @@ static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_top
     if (!dense) {
         rc = orc_lu_multifrontal_f32(t, a32, r32, x32);
         for (int32_t i = 0; i < n; ++i) vn[i] = (double)x32[i];
-        if (lu_kind == ORC_LU_AUTO && t->n <= NR_DENSE_MAX_N) {
+        if (lu_kind == ORC_LU_AUTO && (t->n <= NR_DENSE_MAX_N || promote)) {
             double* ad = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
             double* rd = (double*)xmalloc(sizeof(double) * (size_t)n);
             for (int32_t k = 0; k < t->nnz_a; ++k) ad[k] = (double)a32[k];
@@ -1315,6 +1318,10 @@ static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_top
             dense = rc != NR_OK || orc_lu_flag(t, ad, rd, vn, ORC_LU_ETA_FP32, NULL);
             *fallback = dense;
             free(ad); free(rd);
+            if (dense && t->n > NR_DENSE_MAX_N) {
+                *fallback = 2;
+                return NR_OK;
+            }
         }
     }
     if (dense) {
@@ -1604,7 +1611,8 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     double* res = (double*)xmalloc(sizeof(double) * (size_t)n);
     double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
     double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
-    const int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    const int promote = p32 && o.lu_kind == ORC_LU_AUTO && t->n > NR_DENSE_MAX_N;
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
@@ -1624,12 +1632,18 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
         for (; it < o.max_iter; ++it) {
             int rc;
             int32_t fb = 0;
+            rc = NR_OK;
             if (p32) {
-                rc = newton_step_f32(nl, s, t, v, gmin, NULL, o.lu_kind, vals, av, a32, r32, x32, res, vn, &fb);
-            } else {
+                rc = newton_step_f32(nl, s, t, v, gmin, NULL, o.lu_kind, promote, vals, av, a32, r32, x32, res, vn,
+                                     &fb);
+                if (fb == 2) { p32 = 0; fb = 1; }   /* the rest of the solve on the FP64 path */
+            }
+            if (!p32) {
+                int32_t fb64 = 0;
                 stamp_with_soa(nl, s, t, v, gmin, vals, rhs, NULL);
                 orc_assemble(t, vals, av);
-                rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb);
+                rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb64);
+                fb += fb64;
             }
             out->lu_fallbacks += fb;
             if (rc != NR_OK) {
@@ -2017,7 +2031,8 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
                 int rc;
                 int32_t fb = 0;
                 if (p32) {
-                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, o.lu_kind, vals, av, a32, r32, x32, res, vn, &fb);
+                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, o.lu_kind, 0, vals, av, a32, r32, x32, res, vn,
+                                         &fb);
                 } else {
                     stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
                     orc_assemble(t, vals, av);

@@ -82,7 +82,7 @@ int orc_lu_multifrontal_f32(const orc_topo* t, const float* a_vals, const float*
  * ORC_LU_ETA_FP32 for the FP32 path's float system).  *eta (optional): the
  * ratio, or infinity when a value is not finite. */
 #define ORC_LU_ETA_FP64 1e-8
-#define ORC_LU_ETA_FP32 1e-3
+#define ORC_LU_ETA_FP32 0.1   /* an inexact Newton step: the FP64 residual corrects it */
 int orc_lu_flag(const orc_topo* t, const double* a_vals, const double* rhs, const double* x, double bound,
                 double* eta);
 /* NR_LU_AUTO on one FP64 system: multifrontal, flagged solves (and singular
@@ -108,7 +108,9 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
 /* ORC_LU_AUTO: orc_lu_auto (the product's default).  On the FP32 path,
  * ORC_LU_DENSE (always) and ORC_LU_AUTO (when orc_lu_flag flags the float
  * multifrontal solve) solve the Newton correction with orc_lu_dense on the
- * FP64 Jacobian and the FP64 Newton residual. */
+ * FP64 Jacobian and the FP64 Newton residual.  Above NR_DENSE_MAX_N nodes a
+ * DC solve on the FP32 path with ORC_LU_AUTO takes a flagged iteration again
+ * on the FP64 path and finishes the solve there (one lu_fallback). */
 enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2, ORC_LU_AUTO = 3 };
 /* ORC_PREC_FP64: the reference's FP64 path.  ORC_PREC_FP32_REFINE: the
  * Jacobian (device evaluation, stamps, assembly) and the multifrontal LU in

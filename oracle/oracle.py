@@ -422,6 +422,20 @@ def stamp(nl, topo: Topology, v, gmin):
     return vals, av, rhs
 
 
+def node_conductance(nl, topo: Topology, v, gmin=1e-12):
+    """Diagonal of the DC Jacobian at v (S per node): how firmly the circuit
+    holds each node.  A voltage difference d at node i is a current of about
+    G_ii |d| there, which compares with the KCL acceptance of 1e-9 A."""
+    _, av, _ = stamp(nl, topo, v, gmin)
+    rp = topo.export(_abi.NR_EXPORT_ROW_PTR)
+    ci = topo.export(_abi.NR_EXPORT_COL_IDX)
+    rows = np.repeat(np.arange(topo.n), np.diff(rp))
+    d = np.zeros(topo.n)
+    on = ci == rows
+    d[rows[on]] = av[on]
+    return d
+
+
 def lu_dense(n, rp, ci, vals, rhs):
     x = np.zeros(n)
     sp = C.c_double()

@@ -7,7 +7,9 @@ Monte-Carlo instances (per-device VTO sigma 30 mV, KP sigma 3 %).
 - DC on the full batch: sampled instances bit-identical to the oracle
   (level-1 MOSFETs involve no exp / log), iterations and statuses equal; a
   rerun is bit-identical; an instance solved alone equals its row of the
-  batch; the FP32 + refinement path within the voltage tolerance of FP64.
+  batch.
+- The FP32 path: sampled instances bit-identical to the oracle's; converged
+  answers the same operating point as FP64 to the KCL acceptance.
 - The transient from the operating points: the first steps of sampled
   instances bit-identical to the oracle.
 """
@@ -87,11 +89,16 @@ def test_dc_full_batch(setup):
     bt.close()
 
 
-@pytest.mark.skip(reason="the float multifrontal LU meets a vanishing pivot at gmin <= 1e-11 on "
-                         "this circuit (floating NAND3 stack nodes); FP64 is unaffected")
 def test_dc_fp32_path(setup):
-    """The FP32 path within tolerance of FP64 where both converge."""
+    """The FP32 path: float solves near the root meet vanishing pivots at
+    gmin <= 1e-11 (NAND stack nodes that only off transistors and gmin hold),
+    so instances finish on the FP64 path (lu_fallbacks counts it).  Sampled
+    instances bit-identical to the oracle's FP32 path; converged answers pass
+    the KCL acceptance, and where they differ from FP64 the difference, as a
+    current at each node (G_ii |dv_i|, G = the DC Jacobian's diagonal at the
+    FP64 answer), is within that same 1e-9 A acceptance."""
     from karam_amd import nr
+    from oracle import oracle as O
     ctx, nl, f, T, To, order = setup
     bt = nr.Batch(ctx, T, B)
     bt.set_fields(f)
@@ -100,11 +107,20 @@ def test_dc_fp32_path(setup):
     ok = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
     bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)
     V32, st32 = bt.voltages(), bt.stats()
-    both = ok & (st32["status"] == _abi.NR_STATUS_CONVERGED) & (st32["kcl"] <= 1e-9)
-    assert both.mean() >= 0.9
-    err = np.abs(V32[both] - V[both]) / (1e-9 * np.abs(V[both]) + 1e-12)
-    assert err.max() <= 1.0
     bt.close()
+    ok32 = (st32["status"] == _abi.NR_STATUS_CONVERGED) & (st32["kcl"] <= 1e-9)
+    print(f"logic20k fp32: {ok32.sum()}/{B} converged, promoted {(st32['lu_fallbacks'] > 0).sum()}", flush=True)
+    assert (ok32 == ok).mean() >= 0.99
+    Vo, so = O.solve_dc_batch(nl, To, _sub(f, SAMPLE), len(SAMPLE), threads=len(SAMPLE),
+                              precision=O.PREC_FP32_REFINE)
+    for j, b in enumerate(SAMPLE):
+        np.testing.assert_array_equal(V32[b], Vo[j])
+        assert st32["iterations"][b] == so["iterations"][j] and st32["status"][b] == so["status"][j]
+        assert st32["lu_fallbacks"][b] == so["lu_fallbacks"][j]
+    both = np.flatnonzero(ok & ok32)
+    for b in both[:: max(1, len(both) // 16)]:
+        g = O.node_conductance(netlists.instance(nl, f, b), To, V[b])
+        assert (g * np.abs(V32[b] - V[b])).max() <= 1e-9, b
 
 
 def test_transient_first_steps(setup):

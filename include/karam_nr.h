@@ -165,7 +165,15 @@ enum {
     NR_FIELD_MOS_VTO = 3,    /* [B][n_mos]   */
     NR_FIELD_ISRC_AMPS = 4,  /* [B][n_isrc]  */
     NR_FIELD_VSRC_VOLTS = 5, /* [B][n_vsrc]  */
-    NR_FIELD_CAP_FARADS = 6  /* [B][n_cap]   */
+    NR_FIELD_CAP_FARADS = 6, /* [B][n_cap]   */
+    /* [B][1]: each instance's ambient temperature (K), a corner's
+     * temperature (CornerConfig::temperatureC, physics/corner_config.h:42-95).
+     * It scales the diodes' V_T as tensorizeNetlist does
+     * (tensors/physics_tensors.cpp:419) and again in pnjlim
+     * (solvers/circuitsim_solver.cpp:127-129); unset, every instance takes
+     * the netlist's ambient_temp_K.  Level-1 MOSFETs have no temperature
+     * terms in the reference. */
+    NR_FIELD_TEMP_K = 7
 };
 
 /* ---- opaque handles ------------------------------------------------------ */
@@ -269,9 +277,9 @@ enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
  *                     solveLU_Pivoted (FP32 path: on the FP64 Jacobian and
  *                     the FP64 Newton residual).  Circuits above
  *                     NR_DENSE_MAX_N nodes take the multifrontal solve
- *                     alone; there a DC solve on the FP32 path takes a
- *                     rejected iteration again on the FP64 path and
- *                     finishes the solve there (counted in lu_fallbacks).
+ *                     alone; there a DC solve or transient on the FP32
+ *                     path takes a rejected iteration again on the FP64
+ *                     path and finishes there (counted in lu_fallbacks).
  *  NR_LU_MULTIFRONTAL the multifrontal LU alone.
  *  NR_LU_DENSE        the restated solveLU_Pivoted for every solve: the
  *                     reference's LU bit for bit, O(n^3) per Newton step;

@@ -21,7 +21,8 @@ NR_FIELD_MOS_VTO = 3
 NR_FIELD_ISRC_AMPS = 4
 NR_FIELD_VSRC_VOLTS = 5
 NR_FIELD_CAP_FARADS = 6
-N_FIELDS = 7
+NR_FIELD_TEMP_K = 7
+N_FIELDS = 8
 
 NR_EXPORT_ROW_PTR = 0
 NR_EXPORT_COL_IDX = 1

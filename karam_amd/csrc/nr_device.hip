@@ -197,8 +197,8 @@ struct FinalDiag {
 
 // InstState.done in a transient: 0 solving, 1 finished, 2 the attempt's
 // Newton solve converged, 3 it failed (the step control decides), 4 the
-// next attempt is armed (k_tran_begin sets it up).  DC on the FP32 path: 5
-// the instance leaves the FP32 sweeps to finish on the FP64 path
+// next attempt is armed (k_tran_begin sets it up).  On the FP32 path (DC
+// and transient): 5 the instance leaves the FP32 sweeps to finish on FP64
 enum { DONE_RUNNING = 0, DONE_FINISHED = 1, DONE_CONVERGED = 2, DONE_FAILED = 3, DONE_ARMED = 4, DONE_PROMOTE = 5 };
 
 struct DevBatch {
@@ -229,6 +229,11 @@ struct DevBatch {
     const double *vnow, *inow;
     double *arb_prev, *arb_prev2;
     TranState* ts;
+    // per-instance temperature (NR_FIELD_TEMP_K; null: the netlist's ambient
+    // for every instance): the diodes' V_T scaled to the instance
+    // (interleaved, as the tensorizer scales it) and applyPnJunctionLimits'
+    // V_T and Is factor (instance-major)
+    double *vtI, *pnvt, *pnisf;
     // FP32 path (p32 = 1): the FP64 KCL vector of the current iterate
     // (instance-major); its interleaved copy lives in rhslin, which the FP32
     // path does not otherwise use
@@ -251,10 +256,10 @@ struct DevBatch {
     // dn_slots workspaces of n * n doubles (null when the route does not use
     // the dense LU)
     int lu;
-    // DC on the FP32 path above NR_DENSE_MAX_N nodes with the AUTO route:
-    // an instance whose float solve k_lu_flag rejects (or whose float LU met
-    // a vanishing pivot) drops that iteration and finishes the solve on the
-    // FP64 path from the same iterate
+    // FP32 path above NR_DENSE_MAX_N nodes with the AUTO route (DC and
+    // transient): an instance whose float solve k_lu_flag rejects (or whose
+    // float LU met a vanishing pivot) drops that iteration and finishes on
+    // the FP64 path from the same iterate
     int promote;
     // DC: Newton iterations to take after the last GMIN step converges
     // (nr_solve_opts.polish)
@@ -301,6 +306,13 @@ __device__ __forceinline__ double cmag(cdouble a) {
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ double volt(const double* v, int n, int node) {
     return (node > 0 && node <= n) ? v[node - 1] : 0.0;
+}
+
+// V_T of diode i in instance b: the tensorizer's V_T * T / 300.15
+// (tensors/physics_tensors.cpp:419) at the instance's temperature, or the
+// netlist's ambient
+__device__ __forceinline__ double diode_vt(const DevTopo& T, const DevBatch& P, int i, int b) {
+    return P.vtI ? P.vtI[IX(i, b, P.Bp)] : T.d_vt[i];
 }
 
 // batchDiodePhysics_generic (tensors/physics_tensors.cpp:19-51)
@@ -446,8 +458,11 @@ __device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const
         val = P.inow[(size_t)b * T.nI + idx];
         break;
     case karam::TK_DIODE: {
+        // at the V_T the stamps use: the reference's audit takes the
+        // unscaled V_T (physics/circuitsim.h:1346-1347) and so can never pass
+        // away from 300.15 K, where the scale is exactly 1
         double vd = volt(v, n, T.d_a[idx]) - volt(v, n, T.d_c[idx]);
-        val = diode_i_clamped(vd, P.Is[(size_t)b * T.nD + idx], T.d_n[idx], T.d_vtraw[idx]);
+        val = diode_i_clamped(vd, P.Is[(size_t)b * T.nD + idx], T.d_n[idx], diode_vt(T, P, idx, b));
         break;
     }
     default: {  // TK_MOS
@@ -1072,7 +1087,7 @@ __global__ __launch_bounds__(256) void k_eval(DevTopo T, DevBatch P, const int* 
         const int i = item;
         double vd = V(T.d_a[i]) - V(T.d_c[i]);
         double id, g;
-        diode_eval(vd, P.IsI[IX(i, b, Bp)], T.d_n[i], T.d_vt[i], id, g);
+        diode_eval(vd, P.IsI[IX(i, b, Bp)], T.d_n[i], diode_vt(T, P, i, b), id, g);
         P.dg[IX(i, b, Bp)] = g;
         P.dieq[IX(i, b, Bp)] = id - g * vd;
     } else if (item < T.nD + T.nM) {
@@ -1216,7 +1231,7 @@ __global__ __launch_bounds__(256) void k_eval32(DevTopo T, DevBatch P, const int
     if (item < T.nD) {
         const int i = item;
         reinterpret_cast<float*>(P.dg)[IX(i, b, Bp)] =
-            diode_g32(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], T.d_vt[i]);
+            diode_g32(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], diode_vt(T, P, i, b));
     } else if (item < T.nD + T.nM) {
         const int j = item - T.nD;
         const double vs = V(T.m_s[j]);
@@ -2134,7 +2149,7 @@ __device__ double slot_value64(const DevTopo& T, const DevBatch& P, int b, int q
         double g;
         if (kind == karam::SK_DIODE) {
             double id;
-            diode_eval(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], T.d_vt[i], id, g);
+            diode_eval(V(T.d_a[i]) - V(T.d_c[i]), P.IsI[IX(i, b, Bp)], T.d_n[i], diode_vt(T, P, i, b), id, g);
         } else {
             double vgs, vds, ids, gm, gds;
             mos_eval(V(T.m_g[i]), V(T.m_d[i]), V(T.m_s[i]), T.m_w[i], T.m_l[i], P.kpI[IX(i, b, Bp)],
@@ -2354,8 +2369,8 @@ __global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int
     for (int l = 0; !tran && l < T.n_pn_levels; ++l) {
         for (int k = T.pn_level_ptr[l] + threadIdx.x; k < T.pn_level_ptr[l + 1]; k += blockDim.x) {
             const int i = T.pn_diode[k];
-            const double Vt = T.d_pn_vt[i];
-            const double Is = Isb[i] * T.d_pn_isf[i];
+            const double Vt = P.pnvt ? P.pnvt[(size_t)b * T.nD + i] : T.d_pn_vt[i];
+            const double Is = Isb[i] * (P.pnisf ? P.pnisf[(size_t)b * T.nD + i] : T.d_pn_isf[i]);
             const int p = T.d_a[i] - 1, q = T.d_c[i] - 1;
             const double vcrit = Vt * log(Vt / (sqrt(2.0) * Is));
             const double vjo = (p >= 0 ? v[p] : 0.0) - (q >= 0 ? v[q] : 0.0);

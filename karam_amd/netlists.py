@@ -92,6 +92,7 @@ class Netlist:
             _abi.NR_FIELD_ISRC_AMPS: len(self.isrc_amps),
             _abi.NR_FIELD_VSRC_VOLTS: len(self.vsrc_volts),
             _abi.NR_FIELD_CAP_FARADS: len(self.cap_farads),
+            _abi.NR_FIELD_TEMP_K: 1,
         }[fid]
 
     def card_params(self):
@@ -284,7 +285,10 @@ def instance(nl: Netlist, fields: dict, b: int) -> Netlist:
              _abi.NR_FIELD_MOS_VTO: "mos_vto", _abi.NR_FIELD_ISRC_AMPS: "isrc_amps",
              _abi.NR_FIELD_VSRC_VOLTS: "vsrc_volts", _abi.NR_FIELD_CAP_FARADS: "cap_farads"}
     for fid, arr in fields.items():
-        setattr(out, names[fid], np.array(arr[b], dtype=np.float64))
+        if fid == _abi.NR_FIELD_TEMP_K:
+            out.ambient_temp_K = float(np.asarray(arr, dtype=np.float64).reshape(-1)[b])
+        else:
+            setattr(out, names[fid], np.array(arr[b], dtype=np.float64))
     return out
 
 

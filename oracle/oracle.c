@@ -1470,9 +1470,14 @@ static double kcl_soa(const nr_netlist* nl, const orc_soa* s, const double* v,
         const double I = (tc && tc->isrc) ? tc->isrc[i] : nl->isrc_amps[i];
         ADDRES(nl->isrc_pos[i], -I); ADDRES(nl->isrc_neg[i], I);
     }
+    /* diodes at the V_T the stamps use (V_T * T / 300.15): the reference's
+     * audit takes the unscaled V_T (physics/circuitsim.h:1346-1347), so away
+     * from 300.15 K its own audit can never pass; fixed as D7's MOSFET audit
+     * is.  At 300.15 K the scale is exactly 1. */
+    const double tscale = nl->ambient_temp_K / 300.15;
     for (int32_t i = 0; i < nl->n_diode; ++i) {
         double vd = volt(v, n, nl->dio_anode[i]) - volt(v, n, nl->dio_cathode[i]);
-        double id = diode_i_clamped(vd, nl->dio_is[i], nl->dio_n[i], nl->dio_vt[i]);
+        double id = diode_i_clamped(vd, nl->dio_is[i], nl->dio_n[i], nl->dio_vt[i] * tscale);
         ADDRES(nl->dio_anode[i], -id); ADDRES(nl->dio_cathode[i], id);
     }
     if (s->nM > 0) {
@@ -1754,6 +1759,7 @@ int orc_solve_dc_batch(const nr_netlist* nl, const orc_topo* t, int32_t B,
             if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+            if (fields[NR_FIELD_TEMP_K]) inst.ambient_temp_K = fields[NR_FIELD_TEMP_K][b];
         }
         orc_dc_opts o;
         if (opts) o = *opts; else orc_dc_opts_default(&o);
@@ -1968,7 +1974,8 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
     double* hi0 = (double*)xmalloc(sizeof(double) * (size_t)(nl->n_cap + 1));
     orc_arbiter arb;
     arb.prev = (double*)xmalloc(nb); arb.prev2 = (double*)xmalloc(nb); arb.dvdt = (double*)xmalloc(nb);
-    const int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    int p32 = (o.precision == ORC_PREC_FP32_REFINE);
+    const int promote = p32 && o.lu_kind == ORC_LU_AUTO && t->n > NR_DENSE_MAX_N;
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
@@ -2028,15 +2035,19 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
             int conv = 0, sing = 0;
             int32_t it = 0;
             for (; it < o.max_iter; ++it) {
-                int rc;
+                int rc = NR_OK;
                 int32_t fb = 0;
                 if (p32) {
-                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, o.lu_kind, 0, vals, av, a32, r32, x32, res, vn,
-                                         &fb);
-                } else {
+                    rc = newton_step_f32(nl, s, t, v, 1e-12, &tc, o.lu_kind, promote, vals, av, a32, r32, x32, res,
+                                         vn, &fb);
+                    if (fb == 2) { p32 = 0; fb = 1; }   /* the rest of the transient on the FP64 path */
+                }
+                if (!p32) {
+                    int32_t fb64 = 0;
                     stamp_with_soa(nl, s, t, v, 1e-12, vals, rhs, &tc);
                     orc_assemble(t, vals, av);
-                    rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb);
+                    rc = solve_f64(t, o.lu_kind, av, rhs, vn, &fb64);
+                    fb += fb64;
                 }
                 out->lu_fallbacks += fb;
                 if (rc != NR_OK) { sing = 1; break; }
@@ -2127,6 +2138,7 @@ int orc_solve_transient_batch(const nr_netlist* nl, const orc_topo* t, int32_t B
             if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+            if (fields[NR_FIELD_TEMP_K]) inst.ambient_temp_K = fields[NR_FIELD_TEMP_K][b];
         }
         bad |= orc_solve_transient(&inst, t, &o, v_io + (size_t)b * n, n_probe, probe,
                                    wave_t ? wave_t + (size_t)b * cap : NULL,
@@ -2361,6 +2373,7 @@ int orc_solve_ac_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, const
             if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+            if (fields[NR_FIELD_TEMP_K]) inst.ambient_temp_K = fields[NR_FIELD_TEMP_K][b];
         }
         bad |= orc_solve_ac(&inst, t, v_op + (size_t)b * n, o, n_probe, probe,
                             out + (size_t)b * o->n_points * n_probe * 2, &status[b]) != NR_OK;
@@ -2513,6 +2526,7 @@ int orc_solve_noise_batch(const nr_netlist* nl, const orc_topo* t, int32_t B, co
             if (fields[NR_FIELD_ISRC_AMPS]) inst.isrc_amps = fields[NR_FIELD_ISRC_AMPS] + (size_t)b * nl->n_isrc;
             if (fields[NR_FIELD_VSRC_VOLTS]) inst.vsrc_volts = fields[NR_FIELD_VSRC_VOLTS] + (size_t)b * nl->n_vsrc;
             if (fields[NR_FIELD_CAP_FARADS]) inst.cap_farads = fields[NR_FIELD_CAP_FARADS] + (size_t)b * nl->n_cap;
+            if (fields[NR_FIELD_TEMP_K]) inst.ambient_temp_K = fields[NR_FIELD_TEMP_K][b];
         }
         bad |= orc_solve_noise(&inst, t, v_op + (size_t)b * n, o, psd_out + (size_t)b * o->n_points, &vrms[b],
                                &status[b]) != NR_OK;

@@ -109,8 +109,8 @@ int orc_check_convergence(int32_t n, const double* v_new, const double* v_old,
  * ORC_LU_DENSE (always) and ORC_LU_AUTO (when orc_lu_flag flags the float
  * multifrontal solve) solve the Newton correction with orc_lu_dense on the
  * FP64 Jacobian and the FP64 Newton residual.  Above NR_DENSE_MAX_N nodes a
- * DC solve on the FP32 path with ORC_LU_AUTO takes a flagged iteration again
- * on the FP64 path and finishes the solve there (one lu_fallback). */
+ * DC solve or transient on the FP32 path with ORC_LU_AUTO takes a flagged
+ * iteration again on the FP64 path and finishes there (one lu_fallback). */
 enum { ORC_LU_DENSE = 0, ORC_LU_SPARSE = 1, ORC_LU_MULTIFRONTAL = 2, ORC_LU_AUTO = 3 };
 /* ORC_PREC_FP64: the reference's FP64 path.  ORC_PREC_FP32_REFINE: the
  * Jacobian (device evaluation, stamps, assembly) and the multifrontal LU in

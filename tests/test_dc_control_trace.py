@@ -35,8 +35,14 @@ VT = 0.02585
 
 
 class Circ:
-    def __init__(self, n, res=(), vsrc=(), isrc=(), diodes=()):
+    def __init__(self, n, res=(), vsrc=(), isrc=(), diodes=(), temp=300.15):
         self.n, self.res, self.vsrc, self.isrc, self.diodes = n, list(res), list(vsrc), list(isrc), list(diodes)
+        self.temp = temp
+
+    def vt(self):
+        """The tensorizer's V_T at the circuit's temperature
+        (tensors/physics_tensors.cpp:419)."""
+        return VT * (self.temp / 300.15)
 
     def netlist(self):
         r, v, i, d = self.res, self.vsrc, self.isrc, self.diodes
@@ -133,7 +139,7 @@ def stamp(c, v, gmin):
             b[q - 1] += I
     for a, k, Is in c.diodes:
         vd = V(a) - V(k)
-        i, g = diode_iv(vd, Is)
+        i, g = diode_iv(vd, Is, Vt=c.vt())
         g4(a, k, g)
         ieq = i - g * vd
         if a > 0:
@@ -144,7 +150,10 @@ def stamp(c, v, gmin):
     return A, b
 
 
-def kcl(c, v):
+def kcl(c, v, mutant=None):
+    """The audit; diodes at the stamps' V_T (the "audit_raw" mutant takes the
+    netlist's unscaled V_T, as the reference's audit does)."""
+    vt = VT if mutant == "audit_raw" else c.vt()
     r = np.zeros(c.n + 1)
     V = lambda k: v[k - 1] if k > 0 else 0.0   # noqa: E731
     for p, q, R in c.res:
@@ -159,20 +168,27 @@ def kcl(c, v):
         r[p] -= I
         r[q] += I
     for a, k, Is in c.diodes:
-        i = diode_iv(V(a) - V(k), Is)[0]
+        i = diode_iv(V(a) - V(k), Is, Vt=vt)[0]
         r[a] -= i
         r[k] += i
     return float(np.abs(r[1:]).max())
 
 
-def limit(c, vn, vo, sequential=True):
+def limit(c, vn, vo, sequential=True, once=False):
+    """applyPnJunctionLimits: V_T (the tensor's, already scaled once) is
+    scaled by T / 300.15 again and Is by s^3 exp(1.11 / (V_T / s) - 1.11 / V_T)
+    (solvers/circuitsim_solver.cpp:127-129); the "pn_once" mutant skips that."""
+    s = c.temp / 300.15
+    vt = c.vt() if once else c.vt() * s
     base = vn.copy()
     for a, k, Is in c.diodes:
+        if not once:
+            Is = Is * s ** 3 * math.exp(1.11 / (vt / s) - 1.11 / vt)
         src = vn if sequential else base
-        vcrit = VT * math.log(VT / (math.sqrt(2.0) * Is))
+        vcrit = vt * math.log(vt / (math.sqrt(2.0) * Is))
         jo = (vo[a - 1] if a > 0 else 0.0) - (vo[k - 1] if k > 0 else 0.0)
         jn = (src[a - 1] if a > 0 else 0.0) - (src[k - 1] if k > 0 else 0.0)
-        corr = jn - pnjlim(jn, jo, VT, vcrit)
+        corr = jn - pnjlim(jn, jo, vt, vcrit)
         if abs(corr) > 1e-12:
             if a > 0 and k > 0:
                 vn[a - 1] -= corr * 0.5
@@ -213,14 +229,14 @@ def solve_dc(c, max_iter=50, gmin_steps=19, mutant=None, polish=0):
                 break
             vn = np.linalg.solve(A, b)
             vn[~np.isfinite(vn)] = 0.0
-            limit(c, vn, v, sequential=mutant != "parallel")
-            r = kcl(c, vn)
+            limit(c, vn, v, sequential=mutant != "parallel", once=mutant == "pn_once")
+            r = kcl(c, vn, mutant)
             if it > 0 and ge(r, final) and polished == 0:     # polish iterations take the full step
                 vn = v + 0.5 * (vn - v)
-                r = kcl(c, vn)
+                r = kcl(c, vn, mutant)
                 if mutant != "single" and ge(r, final):
                     vn = v + 0.5 * (vn - v)
-                    r = kcl(c, vn)
+                    r = kcl(c, vn, mutant)
             rec.append((r, float(np.abs(vn - v).max()), 0.5 if (it > 0 and ge(r, final) and polished == 0) else 1.0,
                         k, it))
             final = r
@@ -272,16 +288,27 @@ CIRCUITS = {
     "isrc_diode": (Circ(2, res=[(1, 2, 50.0), (2, 0, 1e5)], isrc=[(0, 1, 0.05)], diodes=[(1, 0, IS)]), {}),
     # two extra Newton iterations after the last GMIN step converges
     "diode_r_polish": (Circ(2, res=[(1, 2, 1e3)], vsrc=[(1, 0, 5.0)], diodes=[(2, 0, IS)]), {"polish": 2}),
+    # corners: the series diodes hot, the current-driven diode cold
+    "two_diodes_400K": (Circ(4, res=[(1, 2, 1e3), (3, 0, 1e4), (2, 4, 1e3)], vsrc=[(1, 0, 5.0)],
+                             diodes=[(2, 3, IS), (3, 0, IS), (0, 4, IS)], temp=400.0), {}),
+    "isrc_diode_250K": (Circ(2, res=[(1, 2, 50.0), (2, 0, 1e5)], isrc=[(0, 1, 0.05)], diodes=[(1, 0, IS)],
+                             temp=250.0), {}),
+    "diode_r_400K": (Circ(2, res=[(1, 2, 1e3)], vsrc=[(1, 0, 5.0)], diodes=[(2, 0, IS)], temp=400.0), {}),
+    "diode_r_250K": (Circ(2, res=[(1, 2, 1e3)], vsrc=[(1, 0, 5.0)], diodes=[(2, 0, IS)], temp=250.0), {}),
     # a resistor with no value: a singular first system
     "singular": (Circ(2, res=[(1, 2, float("nan")), (2, 0, 2e3)], vsrc=[(1, 0, 5.0)]), {}),
 }
 
 
 def _oracle(c, max_iter=50, polish=0):
+    """The oracle's batch solve; a corner's temperature goes in as the
+    per-instance field."""
+    from karam_amd import _abi
     from oracle import oracle as O
     nl = c.netlist()
     T = O.Topology(nl)
-    V, s = O.solve_dc_batch(nl, T, {}, 1, max_iter=max_iter, trace_cap=4096, polish=polish)
+    f = {_abi.NR_FIELD_TEMP_K: np.array([c.temp])} if c.temp != 300.15 else {}
+    V, s = O.solve_dc_batch(nl, T, f, 1, max_iter=max_iter, trace_cap=4096, polish=polish)
     tr = {k: x[0] for k, x in s["trace"].items()}
     h = int(s["history"][0])
     recs = [(tr["residual"][j], tr["max_dv"][j], tr["damping"][j], tr["gmin_step"][j], tr["iteration"][j])
@@ -349,7 +376,7 @@ def test_damping_doubling_is_exercised(oracle_lib):
     assert diffs
 
 
-@pytest.mark.parametrize("mutant", ["gt", "single", "parallel"])
+@pytest.mark.parametrize("mutant", ["gt", "single", "parallel", "audit_raw", "pn_once"])
 def test_mutants_disagree_with_the_oracle(runs, mutant):
     caught = [n for n, (c, kw) in CIRCUITS.items() if not _same(runs[n][0], solve_dc(c, mutant=mutant, **kw))]
     assert caught, mutant
@@ -361,3 +388,22 @@ def test_divider_matches_closed_form(runs):
     v = runs["divider"][1][4]
     G = np.array([[1e3 + 1e-3 + 1e-12, -1e-3], [-1e-3, 1e-3 + 5e-4 + 1e-12]])
     np.testing.assert_allclose(v, np.linalg.solve(G, [5e3, 0.0]), rtol=1e-12)
+
+
+@pytest.mark.parametrize("temp", [250.0, 400.0])
+def test_corner_diode_matches_closed_form(runs, temp):
+    """diode + R at a corner temperature: the answer solves the two node
+    equations with V_T scaled once, I_S (exp(v2 / (N V_T T / 300.15)) - 1) =
+    (v1 - v2) / 1k, the source as 1 mOhm and gmin 1e-12 on both nodes."""
+    from scipy.optimize import brentq
+    v1, v2 = runs[f"diode_r_{int(temp)}K"][0][4]
+    vt = VT * (temp / 300.15)
+    assert runs[f"diode_r_{int(temp)}K"][0][3] == 0
+    # node 1: (5 - v1) / 1e-3 = (v1 - v2) / 1e3 + 1e-12 v1, solved for v1 given v2
+    node1 = lambda x: (5e3 + x * 1e-3) / (1e3 + 1e-3 + 1e-12)   # noqa: E731
+    f = lambda x: IS * math.expm1(x / vt) + 1e-12 * x - (node1(x) - x) / 1e3   # noqa: E731
+    x = brentq(f, 0.0, 1.5, xtol=1e-16, rtol=1e-15)
+    assert abs(v2 - x) <= 1e-9 * abs(x) + 1e-12
+    assert abs(v1 - node1(x)) <= 1e-9 * abs(v1) + 1e-12
+    # the stamp scales V_T but not I_S, so the forward drop rises with T
+    assert (x > 0.7) if temp == 400.0 else (x < 0.6)

@@ -147,3 +147,16 @@ def test_transient_first_steps(setup):
     np.testing.assert_array_equal(W, Wo)
     np.testing.assert_array_equal(stt["iterations"], so["iterations"])
     np.testing.assert_array_equal(stt["status"], so["status"])
+    # the FP32 path: float solves that fail go on to FP64, as in the oracle
+    tr.set_seed(V)
+    tr.transient(h=0.5e-9, steps=20, start=_abi.NR_TRAN_START_SEED, probes=probes,
+                 precision=_abi.NR_PREC_FP32_REFINE)
+    W32, st32 = tr.waveform(), tr.stats()
+    _, Wo32, so32 = O.solve_transient_batch(nl, O.Topology(nl, transient=True, ordering=order), _sub(f, SAMPLE), m,
+                                            h=0.5e-9, steps=20, start=1, v0=V, probes=probes, threads=m,
+                                            precision=O.PREC_FP32_REFINE)
+    np.testing.assert_array_equal(W32, Wo32)
+    np.testing.assert_array_equal(st32["iterations"], so32["iterations"])
+    np.testing.assert_array_equal(st32["status"], so32["status"])
+    np.testing.assert_array_equal(st32["lu_fallbacks"], so32["lu_fallbacks"])
+    tr.close()

@@ -407,6 +407,9 @@ typedef struct nr_autotune_result {
     int32_t fronts[NR_TUNE_BUCKETS];       /* large fronts in the bucket         */
     int32_t full[NR_TUNE_BUCKETS];         /* 1: the whole front in LDS          */
     double ms_before, ms_after;            /* factor + back substitution of the batch */
+    double ms_levels;                      /* the same with the tuned per-level launches */
+    double ms_persistent;                  /* the same with the whole-phase kernels (-1: not offered) */
+    int32_t persistent;                    /* 1: the whole-phase kernels were kept (faster) */
 } nr_autotune_result;
 NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
                              nr_autotune_result* out);
@@ -417,6 +420,20 @@ NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
  * like the tuner's choice it is kept by the topology and changes no result. */
 NR_API int nr_batch_set_launch_class(nr_batch* batch, int32_t precision, int32_t bucket, int32_t team,
                                      int32_t block, int32_t full);
+/* The large-front factorisation and back substitution in one launch each
+ * instead of one launch per elimination-tree level and team class: a block
+ * per instance walks the levels, its four wavefronts taking the level's
+ * fronts in turn, with a block barrier between levels (no dependence between
+ * blocks).  It replaces the per-level launches of solveSparse_LU's numeric
+ * phase (math/sparse_lu.cpp:134-330) and changes no result: every front gets
+ * the same operations in the same order.  on = 0 returns to the per-level
+ * launches.  Kept by the topology (every batch of it, as the launch
+ * classes); FP64 and FP32 separately (precision NR_PREC_FP64 /
+ * NR_PREC_FP32_REFINE); AC and noise keep the per-level launches.
+ * NR_ERR_UNSUPPORTED when the largest front's solution values would not fit
+ * the back substitution's LDS.  nr_batch_autotune also times it against
+ * the tuned per-level launches and keeps the faster. */
+NR_API int nr_batch_set_persistent(nr_batch* batch, int32_t precision, int32_t on);
 
 /* Transient of every instance (the restated stepTransient): capacitor
  * companions of the instance's current method (trapezoidal G = 2C/h with

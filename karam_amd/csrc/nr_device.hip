@@ -145,6 +145,9 @@ struct DevTopo {
     // fronts whose panel is kept in HBM: offset of their panel (front-major),
     // for 8-byte values and for the complex path's 16-byte values
     const long long *sn_gp_off, *sn_gp_off_c;
+    // the whole-phase large-front kernels: HBM panels (front-major) of the
+    // fronts whose panel exceeds a team's LDS share there
+    const long long* sn_pgp_off;
     long long zero_item;
     // transient: capacitors and source waveforms (NR_WAVE_*, 7 values each)
     int nC;
@@ -218,6 +221,7 @@ struct DevBatch {
     double *pool, *Us, *ys, *xs;
     double *Al, *rhsl, *UL, *CBL, *RCBL;
     double* gpanel;              // HBM panels of the fronts too large for LDS
+    double* pgpanel;             // the same for the whole-phase kernels (k_factor_persist)
     double* seed;                // optional [B][n]
     // transient: capacitances, capacitor history (branch voltages of the
     // last two accepted steps, current of the last), source values at the
@@ -1573,36 +1577,17 @@ __device__ __forceinline__ void team_sync() {
 //       4 x 2 register tiles and streamed to HBM, its entries gathered straight
 //       from the children's update blocks (every A entry of a front lies in a
 //       pivot row or column, so only children contribute).
-template <int TEAM, int NT, bool GP, typename V>
-__global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
-                                                     const FrontRec* __restrict__ fr_list, int n_sn, int lds_team) {
+// One front of one instance, by a team of TEAM lanes (lane = its index in
+// the team) whose panel Pr (np = c r + m c + r values) lies in LDS or in
+// HBM.  The panel is cleared first.  Every entry receives the same
+// operations in the same order whatever TEAM is.
+template <int TEAM, typename V>
+__device__ __forceinline__ void factor_front(const DevTopo& T, const DevBatch& P, const FrontRec& fe, int b, int lane,
+                                             V* Pr) {
 #pragma clang fp contract(off)
-    // a team wider than a wavefront synchronises the whole block, so it must
-    // be alone in it (early exits and per-front barrier counts then agree)
-    static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
-    // eager launches pass the active count; graph replays pass minus the
-    // grid's count and read the live one (at most that) from the device
-    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    V* smem = reinterpret_cast<V*>(smem_raw);
-    constexpr int TPB = NT / TEAM;
-    const int tib = threadIdx.x / TEAM;
-    const int lane = threadIdx.x % TEAM;
-    // tasks are front-major: the teams sharing a wavefront factor the same
-    // front for consecutive instances, so their control flow is identical
-    constexpr int TPW = TEAM < 64 ? 64 / TEAM : 1;
-    const int nact_pad = (nact + TPW - 1) / TPW * TPW;
-    const long long task = (long long)blockIdx.x * TPB + tib;
-    if (task >= (long long)n_sn * nact_pad) return;
-    const FrontRec fe = fr_list[task / nact_pad];
-    const int J = fe.J;
-    const int slot = (int)(task % nact_pad);
-    if (slot >= nact) return;
-    const int b = act[slot];
     const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
     const int np = c * r + m * c + r;                 // panel size
-    V* Pr = GP ? vp<V>(P.gpanel) + fblk(gp_offsets<V>(T)[J], b, np, P.B)
-               : smem + (size_t)tib * (size_t)lds_team;   // pivot rows, c x r (row stride r)
+    // pivot rows, c x r (row stride r);
     // front-local (i, j) with i < c or j < c; rows c.. of the pivot columns
     // have stride c in the panel
     const int ls = c;
@@ -1776,6 +1761,37 @@ __global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, cons
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
+// One launch per level and team class: a team per (front, instance)
+template <int TEAM, int NT, bool GP, typename V>
+__global__ __launch_bounds__(NT) void k_factor_large(DevTopo T, DevBatch P, const int* __restrict__ act, int nact_grid,
+                                                     const FrontRec* __restrict__ fr_list, int n_sn, int lds_team) {
+    // a team wider than a wavefront synchronises the whole block, so it must
+    // be alone in it (early exits and per-front barrier counts then agree)
+    static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
+    // eager launches pass the active count; graph replays pass minus the
+    // grid's count and read the live one (at most that) from the device
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    V* smem = reinterpret_cast<V*>(smem_raw);
+    constexpr int TPB = NT / TEAM;
+    const int tib = threadIdx.x / TEAM;
+    const int lane = threadIdx.x % TEAM;
+    // tasks are front-major: the teams sharing a wavefront factor the same
+    // front for consecutive instances, so their control flow is identical
+    constexpr int TPW = TEAM < 64 ? 64 / TEAM : 1;
+    const int nact_pad = (nact + TPW - 1) / TPW * TPW;
+    const long long task = (long long)blockIdx.x * TPB + tib;
+    if (task >= (long long)n_sn * nact_pad) return;
+    const FrontRec fe = fr_list[task / nact_pad];
+    const int slot = (int)(task % nact_pad);
+    if (slot >= nact) return;
+    const int b = act[slot];
+    const int np = fe.c * (fe.c + fe.m) + fe.m * fe.c + fe.c + fe.m;
+    V* Pr = GP ? vp<V>(P.gpanel) + fblk(gp_offsets<V>(T)[fe.J], b, np, P.B)
+               : smem + (size_t)tib * (size_t)lds_team;
+    factor_front<TEAM, V>(T, P, fe, b, lane, Pr);
+}
+
 // Large fronts with the whole front in LDS, one team per (front, instance):
 // the r x r front and its rhs are assembled there (the A entries, then each
 // child's update block in child order, read once in its own row-major order
@@ -1885,11 +1901,39 @@ __global__ __launch_bounds__(NT) void k_factor_full(DevTopo T, DevBatch P, const
     if (sing && lane == 0) P.st[b].singular = 1;
 }
 
+// Backward substitution of one front of one instance by a team of TEAM
+// lanes, with the front's r solution values staged in xl (LDS)
+template <int TEAM, typename V>
+__device__ __forceinline__ void backsolve_front(const DevTopo& T, const DevBatch& P, const FrontRec& fe, int b,
+                                                int lane, V* xl) {
+#pragma clang fp contract(off)
+    const int Bp = P.Bp;
+    const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
+    const V* U = vp<V>(P.UL) + fblk(fe.u_off, b, c * r, P.B);
+    const int* rows = T.row_idx + fe.row_off;
+    V* xs = vp<V>(P.xs);
+    const V* ys = vp<V>(P.ys);
+    for (int a = lane; a < m; a += TEAM) xl[c + a] = xs[IX(rows[a], b, Bp)];
+    team_sync<TEAM>();
+    for (int k = lane; k < c; k += TEAM) {
+        V s = ys[IX(f + k, b, Bp)];
+        const V* Uk = U + (size_t)k * r + c;
+        for (int a = 0; a < m; ++a) s = vfma(-Uk[a], xl[c + a], s);
+        xl[k] = s;
+    }
+    team_sync<TEAM>();
+    for (int k = c - 1; k >= 0; --k) {
+        const V xk = xl[k] / U[(size_t)k * r + k];
+        for (int i = lane; i < k; i += TEAM) xl[i] = vfma(-U[(size_t)i * r + k], xk, xl[i]);
+        if (lane == 0) xs[IX(f + k, b, Bp)] = xk;
+        team_sync<TEAM>();
+    }
+}
+
 template <int TEAM, int NT, typename V>
 __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, const int* __restrict__ act,
                                                         int nact_grid, const FrontRec* __restrict__ fr_list, int n_sn,
                                                         int rmax) {
-#pragma clang fp contract(off)
     // a team wider than a wavefront synchronises the whole block, so it must
     // be alone in it (early exits and per-front barrier counts then agree)
     static_assert(TEAM <= 64 || NT == TEAM, "teams above 64 lanes need one team per block");
@@ -1910,28 +1954,63 @@ __global__ __launch_bounds__(NT) void k_backsolve_large(DevTopo T, DevBatch P, c
     const FrontRec fe = fr_list[task / nact_pad];
     const int slot = (int)(task % nact_pad);
     if (slot >= nact) return;
-    const int b = act[slot];
-    const int Bp = P.Bp;
-    const int c = fe.c, m = fe.m, r = c + m, f = fe.f;
-    V* xl = smem + (size_t)tib * (size_t)rmax;
-    const V* U = vp<V>(P.UL) + fblk(fe.u_off, b, c * r, P.B);
-    const int* rows = T.row_idx + fe.row_off;
-    V* xs = vp<V>(P.xs);
-    const V* ys = vp<V>(P.ys);
-    for (int a = lane; a < m; a += TEAM) xl[c + a] = xs[IX(rows[a], b, Bp)];
-    team_sync<TEAM>();
-    for (int k = lane; k < c; k += TEAM) {
-        V s = ys[IX(f + k, b, Bp)];
-        const V* Uk = U + (size_t)k * r + c;
-        for (int a = 0; a < m; ++a) s = vfma(-Uk[a], xl[c + a], s);
-        xl[k] = s;
+    backsolve_front<TEAM, V>(T, P, fe, act[slot], lane, smem + (size_t)tib * (size_t)rmax);
+}
+
+// ---------------------------------------------------------------------------
+// kernels: the whole large-front phase in one launch
+// ---------------------------------------------------------------------------
+// Every large front of an instance, level by level, by one block: each of
+// its four wavefronts is a 64-lane team that takes the level's fronts in turn,
+// and a block barrier separates the levels, so a front's children are
+// complete and visible to the block before it starts (workgroup scope: no
+// waits across blocks).  Blocks loop over the active instances.  A front
+// whose panel exceeds the team's LDS share lds_team keeps its panel in HBM
+// (sn_pgp_off).  The results are those of the per-level launches: the
+// arithmetic of a front does not depend on the team.
+template <typename V>
+__global__ __launch_bounds__(256) void k_factor_persist(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                        int nact_grid, const FrontRec* __restrict__ list,
+                                                        const int* __restrict__ lev_ptr, int n_lev, int lds_team) {
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    V* Pl = reinterpret_cast<V*>(smem_raw) + (size_t)wave * (size_t)lds_team;
+    for (int slot = blockIdx.x; slot < nact; slot += gridDim.x) {
+        const int b = act[slot];
+        for (int l = 0; l < n_lev; ++l) {
+            const int p1 = lev_ptr[l + 1];
+            for (int p = lev_ptr[l] + wave; p < p1; p += 4) {
+                const FrontRec fe = list[p];
+                const int np = fe.c * (fe.c + fe.m) + fe.m * fe.c + fe.c + fe.m;
+                if (np <= lds_team)
+                    factor_front<64, V>(T, P, fe, b, lane, Pl);
+                else
+                    factor_front<64, V>(T, P, fe, b, lane,
+                                        vp<V>(P.pgpanel) + fblk(T.sn_pgp_off[fe.J], b, np, P.B));
+            }
+            __syncthreads();
+        }
     }
-    team_sync<TEAM>();
-    for (int k = c - 1; k >= 0; --k) {
-        const V xk = xl[k] / U[(size_t)k * r + k];
-        for (int i = lane; i < k; i += TEAM) xl[i] = vfma(-U[(size_t)i * r + k], xk, xl[i]);
-        if (lane == 0) xs[IX(f + k, b, Bp)] = xk;
-        team_sync<TEAM>();
+}
+
+// The backward substitution of every large front of an instance, levels in
+// reverse, by one block (as k_factor_persist); rmax = the largest front order
+template <typename V>
+__global__ __launch_bounds__(256) void k_backsolve_persist(DevTopo T, DevBatch P, const int* __restrict__ act,
+                                                           int nact_grid, const FrontRec* __restrict__ list,
+                                                           const int* __restrict__ lev_ptr, int n_lev, int rmax) {
+    const int nact = nact_grid > 0 ? nact_grid : *P.nact;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    V* xl = reinterpret_cast<V*>(smem_raw) + (size_t)wave * (size_t)rmax;
+    for (int slot = blockIdx.x; slot < nact; slot += gridDim.x) {
+        const int b = act[slot];
+        for (int l = n_lev - 1; l >= 0; --l) {
+            const int p1 = lev_ptr[l + 1];
+            for (int p = lev_ptr[l] + wave; p < p1; p += 4) backsolve_front<64, V>(T, P, list[p], b, lane, xl);
+            __syncthreads();
+        }
     }
 }
 

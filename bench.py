@@ -382,6 +382,16 @@ def logic_workload(a):
         f"levels={T.info['n_levels']} max_front={T.info['max_front']} symbolic {symbolic_ms:.0f} ms")
     bt = nr.Batch(ctx, T, B)
     bt.set_fields(f)
+    tune = {}
+    if not a.no_tune:                 # untimed: launch shapes and the whole-phase kernels, per precision
+        for key, prec in (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE)):
+            t0 = time.perf_counter()
+            tu = bt.autotune(precision=prec, reps=2)
+            tune[key] = {"s": time.perf_counter() - t0, "factor_solve_ms_before": tu["ms_before"],
+                         "factor_solve_ms_levels": tu["ms_levels"], "factor_solve_ms_persistent": tu["ms_persistent"],
+                         "persistent": tu["persistent"], "factor_solve_ms_after": tu["ms_after"]}
+            log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution "
+                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f})")
     for _ in range(a.warmup):
         bt.solve_dc()
     rate, ms, kst, st, r = timed_solves(bt, a.steps)
@@ -434,7 +444,9 @@ def logic_workload(a):
                                f"backward-Euler transient, FP64 and FP32 + FP64 refinement",
                    "netlist": kw, "instances": B, "nodes": T.info["n"], "mosfets": T.info["n_mos"],
                    "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
-                   "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": r["batched_iterations"]},
+                   "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": r["batched_iterations"],
+                   "supernodes": T.info["n_supernodes"], "levels": T.info["n_levels"]},
+        "autotune": tune or None,
         "converged_fp64": int(ok64.sum()),
         "nr_iters_per_solve": {"plain": iter_stats(it[ok64]) if ok64.any() else None,
                                "plain_fp32_refine": iter_stats(st32["iterations"][ok32]) if ok32.any() else None},
@@ -878,7 +890,7 @@ def main():
     tune = {}
     precs = (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE))
     if a.retune:                      # untimed: launch shapes per front-order bucket, per precision
-        table = {}
+        table, persistent = {}, {}
         for key, prec in precs:
             t0 = time.perf_counter()
             tu = bt.autotune(precision=prec, reps=3)
@@ -888,18 +900,25 @@ def main():
                                       "full" if b["full"] else "panel"]
                                      for b in tu["buckets"]]}
             table[key] = [[b["team"], b["block"], b["full"]] for b in tu["buckets"]]
-            log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution")
+            persistent[key] = int(tu["persistent"])
+            tune[key].update({"factor_solve_ms_levels": tu["ms_levels"],
+                              "factor_solve_ms_persistent": tu["ms_persistent"], "persistent": tu["persistent"]})
+            log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution "
+                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f})")
         with open(a.tune_table, "w") as fo:
-            json.dump({"workload": f"rdmesh{a.side}, {B} instances", "classes": table}, fo, indent=1)
+            json.dump({"workload": f"rdmesh{a.side}, {B} instances", "classes": table, "persistent": persistent},
+                      fo, indent=1)
         tune["table_written"] = os.path.relpath(a.tune_table, ROOT)
     elif not a.no_tune and os.path.exists(a.tune_table):
         # the committed table from a tuning run on MI355X: every run (and every
         # profile of it) uses the same launch class per front-order bucket
-        table = json.load(open(a.tune_table))["classes"]
+        saved = json.load(open(a.tune_table))
+        table, persistent = saved["classes"], saved.get("persistent", {})
         for key, prec in precs:
             for bucket, (team, block, full) in enumerate(table[key]):
                 bt.set_launch_class(bucket, team, block, full=full, precision=prec)
-            tune[key] = {"buckets": table[key]}
+            bt.set_persistent(bool(persistent.get(key, 0)), precision=prec)
+            tune[key] = {"buckets": table[key], "persistent": int(persistent.get(key, 0))}
         tune["table"] = os.path.relpath(a.tune_table, ROOT)
         log(f"launch classes from {tune['table']}")
     for w in range(a.warmup):

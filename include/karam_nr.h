@@ -233,7 +233,10 @@ enum {
     NR_EXPORT_FILL_ROW_PTR = 5,  /* int32[n+1] filled L+U pattern            */
     NR_EXPORT_FILL_COL_IDX = 6,  /* int32[nnz_f]                             */
     NR_EXPORT_DIODE_ORDER = 7,   /* int32[n_diode] SoA pos -> netlist index  */
-    NR_EXPORT_MOS_ORDER = 8      /* int32[n_mos]  SoA pos -> netlist index   */
+    NR_EXPORT_MOS_ORDER = 8,     /* int32[n_mos]  SoA pos -> netlist index   */
+    NR_EXPORT_SN_FIRST = 9       /* int32[n_supernodes] first (permuted)
+                                    column of each relaxed supernode of the
+                                    multifrontal LU (no reference counterpart) */
 };
 NR_API int nr_topology_export(const nr_topology* topo, int32_t what,
                               int32_t* out, int64_t capacity, int64_t* written);

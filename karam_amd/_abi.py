@@ -33,6 +33,7 @@ NR_EXPORT_FILL_ROW_PTR = 5
 NR_EXPORT_FILL_COL_IDX = 6
 NR_EXPORT_DIODE_ORDER = 7
 NR_EXPORT_MOS_ORDER = 8
+NR_EXPORT_SN_FIRST = 9
 
 NR_PREC_FP64 = 0
 NR_PREC_FP32_REFINE = 1

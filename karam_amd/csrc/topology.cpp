@@ -570,15 +570,42 @@ int build_topology(const nr_netlist* nl, Topology& t, std::string& err, bool tra
         t.nnz_f = (int32_t)t.fcol_idx.size();
     }
 
-    // ---- fundamental supernodes --------------------------------------------------
+    // ---- relaxed supernodes -------------------------------------------------------
+    // Consecutive columns along an elimination-tree chain share a front.
+    // Scanning from the last column down, column j-1 joins the supernode
+    // [j, last] when j is its parent and either the pair is fundamental
+    // (|struct(j-1)| = |struct(j)| + 1, j has no other child) or the merged
+    // front's explicit zeros stay within a fifth of its L entries, at front
+    // order <= AMALG_MAX_ORDER and at most AMALG_MAX_PIVOTS pivots (the
+    // elimination of a front's pivot block is sequential in its pivots).  A column's structure lies within the later
+    // columns of its supernode and the last column's rows (etree property),
+    // so the front [first..last] + struct(last) holds every entry.
     {
+        constexpr int64_t AMALG_MAX_ORDER = 512, AMALG_MAX_PIVOTS = 16;
         std::vector<int32_t> nchild(n, 0);
         for (int32_t j = 0; j < n; ++j) if (parent[j] >= 0) nchild[parent[j]]++;
         std::vector<int32_t> start;
-        start.push_back(0);
-        for (int32_t j = 1; j < n; ++j) {
-            bool merge = parent[j - 1] == j && cs[j - 1].size() == cs[j].size() + 1 && nchild[j] == 1;
-            if (!merge) start.push_back(j);
+        {
+            std::vector<char> begins(n, 0);
+            int32_t last = n - 1;
+            int64_t zeros = 0;
+            for (int32_t j = n - 1; j >= 1; --j) {
+                bool join = false;
+                if (parent[j - 1] == j) {
+                    const int64_t cl = (int64_t)cs[last].size();
+                    const int64_t c2 = last - j + 2, r2 = c2 + cl;
+                    const int64_t zadd = (last - j + 1 + cl) - (int64_t)cs[j - 1].size();
+                    const int64_t ents = c2 * r2 - c2 * (c2 + 1) / 2;
+                    if (zadd == 0 && nchild[j] == 1) join = true;
+                    else if (r2 <= AMALG_MAX_ORDER && c2 <= AMALG_MAX_PIVOTS && 5 * (zeros + zadd) <= ents) {
+                        join = true;
+                        zeros += zadd;
+                    }
+                }
+                if (!join) { begins[j] = 1; last = j - 1; zeros = 0; }
+            }
+            start.push_back(0);
+            for (int32_t j = 1; j < n; ++j) if (begins[j]) start.push_back(j);
         }
         start.push_back(n);
         const int32_t ns = (int32_t)start.size() - 1;

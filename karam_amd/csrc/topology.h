@@ -3,7 +3,7 @@
 // Produces everything the batched GPU Newton loop needs that depends only on
 // the circuit's structure: the reference SoA device order, the DC stamp
 // sequence and its CSR pattern / stamp->slot map, the Min-Fill elimination
-// ordering, the filled L+U pattern, fundamental supernodes with their
+// ordering, the filled L+U pattern, relaxed supernodes with their
 // frontal index maps, and the per-node incidence lists used by assembly, the
 // KCL residual and the pn-junction limiter.  Integer outputs that have a
 // reference counterpart are bit-exact with it.

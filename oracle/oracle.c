@@ -285,7 +285,7 @@ struct orc_topo {
     int32_t *perm, *iperm;
     int32_t nnz_f, fill_in;
     int32_t *frow_ptr, *fcol_idx, *fdiag;
-    /* fundamental supernodes of the fill pattern (multifrontal LU) */
+    /* relaxed supernodes of the fill pattern (multifrontal LU) */
     int32_t ns;
     int32_t *sn_first, *sn_ncols, *sn_nrows, *sn_row_off, *sn_rows;
     int32_t *sn_child_ptr, *sn_child;
@@ -698,10 +698,10 @@ static void symbolic(orc_topo* t) {
     free(rows);
 }
 
-/* Fundamental supernodes of the filled pattern: column j-1 joins column j
- * when parent(j-1) = j, |struct L(:,j-1)| = |struct L(:,j)| + 1 and j has no
- * other child.  Children are listed in increasing supernode index; the
- * front of supernode J is ordered [own columns, then its update rows R]. */
+/* Relaxed supernodes of the filled pattern (see supernodes()): consecutive
+ * columns along an elimination-tree chain.  Children are listed in
+ * increasing supernode index; the front of supernode J is ordered [own
+ * columns, then its update rows R = struct L(:, last column)]. */
 static int32_t front_local(const orc_topo* t, int32_t J, int32_t x) {
     int32_t f = t->sn_first[J], c = t->sn_ncols[J];
     if (x >= f && x < f + c) return x - f;
@@ -721,12 +721,35 @@ static void supernodes(orc_topo* t) {
         par[j] = cnt[j] > 0 ? t->fcol_idx[t->fdiag[j] + 1] : -1;
         if (par[j] >= 0) nch[par[j]]++;
     }
+    /* relaxed supernodes: from the last column down, column j-1 joins the
+     * supernode [j, last] when par(j-1) = j and either the pair is
+     * fundamental (no explicit zero, j has no other child) or the merged
+     * front's explicit zeros stay within a fifth of its L entries at front
+     * order <= 512 and at most 16 pivots */
     int32_t* start = (int32_t*)xmalloc(sizeof(int32_t) * (size_t)(n + 1));
+    char* begins = (char*)xcalloc((size_t)n + 1, 1);
+    {
+        int32_t last = n - 1;
+        int64_t zeros = 0;
+        for (int32_t j = n - 1; j >= 1; --j) {
+            int join = 0;
+            if (par[j - 1] == j) {
+                const int64_t cl = cnt[last];
+                const int64_t c2 = last - j + 2, r2 = c2 + cl;
+                const int64_t zadd = (last - j + 1 + cl) - cnt[j - 1];
+                const int64_t ents = c2 * r2 - c2 * (c2 + 1) / 2;
+                if (zadd == 0 && nch[j] == 1) join = 1;
+                else if (r2 <= 512 && c2 <= 16 && 5 * (zeros + zadd) <= ents) { join = 1; zeros += zadd; }
+            }
+            if (!join) { begins[j] = 1; last = j - 1; zeros = 0; }
+        }
+    }
     int32_t ns = 0;
     start[ns++] = 0;
     for (int32_t j = 1; j < n; ++j)
-        if (!(par[j - 1] == j && cnt[j - 1] == cnt[j] + 1 && nch[j] == 1)) start[ns++] = j;
+        if (begins[j]) start[ns++] = j;
     start[ns] = n;
+    free(begins);
     t->ns = ns;
     t->sn_first = (int32_t*)xmalloc(4 * (size_t)ns); t->sn_ncols = (int32_t*)xmalloc(4 * (size_t)ns);
     t->sn_nrows = (int32_t*)xmalloc(4 * (size_t)ns); t->sn_row_off = (int32_t*)xmalloc(4 * (size_t)ns);
@@ -902,6 +925,7 @@ int64_t orc_topology_export(const orc_topo* t, int32_t what, int32_t* out, int64
     case NR_EXPORT_FILL_COL_IDX: src = t->fcol_idx; cnt = t->nnz_f; break;
     case NR_EXPORT_DIODE_ORDER: src = t->soa->d_src; cnt = t->soa->nD; break;
     case NR_EXPORT_MOS_ORDER: src = t->soa->m_src; cnt = t->soa->nM; break;
+    case NR_EXPORT_SN_FIRST: src = t->sn_first; cnt = t->ns; break;
     case ORC_EXPORT_STAMP_ROW: src = t->st_row; cnt = t->n_stamps; break;
     case ORC_EXPORT_STAMP_COL: src = t->st_col; cnt = t->n_stamps; break;
     default: return -1;

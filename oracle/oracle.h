@@ -65,7 +65,7 @@ int orc_lu_dense(int32_t n, const int32_t* rp, const int32_t* ci,
 /* Same-pattern sparse LU on the symbolic fill pattern, no pivoting. */
 int orc_lu_sparse(const orc_topo* t, const double* a_vals, const double* rhs,
                   double* x, double* smallest_pivot);
-/* Multifrontal LU on the fundamental supernodes of the same fill pattern,
+/* Multifrontal LU on the relaxed supernodes of the same fill pattern,
  * no pivoting, with one fixed operation order (assembly: A entry, then the
  * children's update blocks in child order; elimination: explicit FMAs).
  * The HIP library performs exactly these operations in this order. */

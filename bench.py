@@ -714,15 +714,20 @@ def hightw_workload(a):
 def mixed_workload(a):
     """Uneven sizes: a.mixed_instances instances whose node counts are
     log-uniform in [200, 50000], the family of instance i being i mod 3 (the
-    R-diode mesh, the inverter chain, random inverter logic).  Node counts are
-    bucketed into 8 log-spaced sizes per family; each bucket is one circuit
-    with its own sparsity pattern and ordering, solved as one batch of
+    R-diode mesh, the inverter chain, random NAND2/NAND3/INV logic at the
+    logic configuration's depth, about n / 31 gates per level).  Node counts
+    are bucketed into 8 log-spaced sizes per family; each bucket is one
+    circuit with its own sparsity pattern and ordering, solved as one batch of
     Monte-Carlo variants: DC operating points on the FP32 path with hipGraph
     sweeps, launch shapes autotuned per circuit (untimed).  The circuits are
     dealt to a.mixed_streams contexts (one HIP stream each), largest first to
     the least loaded, and solved concurrently from as many host threads, so
     that small circuits run beside the large ones.  Symbolic analysis and
-    tuning happen before the timed region and are reported separately."""
+    tuning happen before the timed region and are reported separately.
+    After the timed run, one eager solve per circuit with per-kernel events
+    gives the roofline of the dominant kernel over the whole workload, and
+    the oracle times one instance of each circuit on its own host thread for
+    the CPU baseline."""
     from concurrent.futures import ThreadPoolExecutor
     from karam_amd import _abi, netlists, nr
     rng = np.random.default_rng(12345)
@@ -750,7 +755,8 @@ def mixed_workload(a):
                 nl = netlists.inverter_chain(max(1, int(round(nrep)) - 2))
                 kw = dict(sigma_vto=0.03, sigma_kp=0.03)
             else:
-                nl = netlists.random_logic(int(round(nrep)), seed=200 + b, p_inv=1.0)
+                nn = int(round(nrep))
+                nl = netlists.random_logic(nn, seed=200 + b, width=max(32, nn // 31), p_inv=0.3, p_nand3=0.1)
                 kw = dict(sigma_vto=0.03, sigma_kp=0.03)
             jobs.append({"fam": fam, "b": b, "cnt": cnt, "nl": nl, "kw": kw, "nodes": nl.num_nodes})
     # largest first to the least loaded context (load ~ nodes x instances)
@@ -766,8 +772,9 @@ def mixed_workload(a):
         j["T"] = nr.Topology(j["nl"], ctx)
         j["symbolic_s"] = time.perf_counter() - t0
         symb_s += j["symbolic_s"]
+        j["f"] = netlists.mc_fields(j["nl"], j["cnt"], seed=1000 + 37 * j["b"] + j["fam"], **j["kw"])
         j["bt"] = nr.Batch(ctx, j["T"], j["cnt"])
-        j["bt"].set_fields(netlists.mc_fields(j["nl"], j["cnt"], seed=1000 + 37 * j["b"] + j["fam"], **j["kw"]))
+        j["bt"].set_fields(j["f"])
         t0 = time.perf_counter()
         if not a.no_tune:
             j["bt"].autotune(precision=prec, reps=2)
@@ -786,19 +793,31 @@ def mixed_workload(a):
     with ThreadPoolExecutor(max_workers=S) as ex:
         list(ex.map(run, range(S)))
     solve_s = time.perf_counter() - t0
+    # per-kernel events: one eager solve per circuit (untimed), summed
+    kst = {}
+    for j in jobs:
+        j["bt"].solve_dc(precision=prec, record_timing=2)
+        for k, x in j["bt"].kernel_stats().items():
+            y = kst.setdefault(k, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
+            for q in y:
+                y[q] += x[q]
     rows, conv_total = [], 0
     for j in jobs:
         st, T = j["st"], j["T"]
-        conv = int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+        ok = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
+        j["ok"] = ok
+        j["Vfirst"] = j["bt"].voltages()[0]
+        j["bt"].close()
+        conv = int(ok.sum())
         conv_total += conv
         rows.append({"family": fams[j["fam"]], "nodes": T.info["n"], "instances": j["cnt"], "converged": conv,
                      "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"], "stream": j["w"],
                      "solve_ms": j["solve_s"] * 1e3, "symbolic_ms": j["symbolic_s"] * 1e3,
                      "sweeps": j["r"]["batched_iterations"], "iterations_mean": float(st["iterations"].mean()),
+                     "finished_on_fp64": int((st["lu_fallbacks"] > 0).sum()),
                      "status_counts": np.bincount(st["status"], minlength=3).tolist()})
         log(f"{fams[j['fam']]} n={T.info['n']} B={j['cnt']} stream {j['w']}: {j['solve_s'] * 1e3:.0f} ms, "
             f"converged {conv}")
-        j["bt"].close()
     per_fam = {}
     for r_ in rows:
         x = per_fam.setdefault(r_["family"], {"instances": 0, "converged": 0, "solve_ms": 0.0})
@@ -819,10 +838,72 @@ def mixed_workload(a):
         "converged": conv_total, "symbolic_s": symb_s, "tune_s": tune_s,
         "value_with_symbolic": conv_total / (solve_s + symb_s),
         "circuit_solve_s_summed": sum(r_["solve_ms"] for r_ in rows) / 1e3,
-        "families": per_fam, "circuits": rows, "roofline": None, "cpu_baseline": None,
+        "families": per_fam, "circuits": rows,
+        "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst),
+        "cpu_baseline": None,
     }
+    if not a.no_cpu:
+        out["cpu_baseline"], out["parity_sample"] = mixed_cpu_baseline(a, jobs, prec)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def mixed_cpu_baseline(a, jobs, prec):
+    """The oracle on one instance of each circuit, each circuit on its own
+    host thread (all at once), its FP32 path as the GPU ran; the workload's
+    CPU time on C cores is sum(instances x seconds per instance) / C.  The
+    inverter chains above 600 nodes are not run: their Newton steps take the
+    dense O(n^3) fallback (up to 4096 nodes), so their time is extrapolated
+    from the largest chain run by (n / n_run)^3 per Newton iteration, with
+    the iterations the GPU took.  The sampled instances are also the parity
+    sample."""
+    from concurrent.futures import ThreadPoolExecutor
+    from karam_amd import _abi
+    from oracle import oracle as O
+    threads = cpu_threads(a)
+    oprec = O.PREC_FP32_REFINE if prec == _abi.NR_PREC_FP32_REFINE else O.PREC_FP64
+    dense_chain = lambda j: j["fam"] == 1 and 600 < j["nodes"] <= 4096  # noqa: E731
+    run = [j for j in jobs if not dense_chain(j)]
+
+    def one(j):
+        To = O.Topology(j["nl"], ordering=j["T"].export(_abi.NR_EXPORT_ORDERING_RAW))
+        t0 = time.perf_counter()
+        Vo, so = O.solve_dc_batch(j["nl"], To, {k: v[:1] for k, v in j["f"].items()}, 1, threads=1, precision=oprec)
+        return time.perf_counter() - t0, Vo[0], so
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, run))
+    wall = time.perf_counter() - t0
+    per = {}
+    for j, (dt, Vo, so) in zip(run, res):
+        per[id(j)] = dt
+    chains = [j for j in run if j["fam"] == 1 and j["nodes"] <= 600]
+    jref = max(chains, key=lambda j: j["nodes"]) if chains else None
+    for j in jobs:
+        if id(j) not in per and jref is not None:
+            it_ref = float(jref["st"]["iterations"][0])
+            it_j = float(j["st"]["iterations"].mean())
+            per[id(j)] = per[id(jref)] / max(it_ref, 1.0) * it_j * (j["nodes"] / jref["nodes"]) ** 3
+    total = sum(j["cnt"] * per[id(j)] for j in jobs)
+    N = sum(j["cnt"] for j in jobs)
+    parity = []
+    for j, (dt, Vo, so) in zip(run, res):
+        Vg = j["Vfirst"]
+        err = float((np.abs(Vg - Vo) / (1e-9 * np.abs(Vo) + 1e-12)).max())
+        parity.append({"family": ("rdmesh", "chain", "logic")[j["fam"]], "nodes": j["nodes"],
+                       "oracle_s": dt, "err_over_tol": err, "bit_identical": bool(np.array_equal(Vg, Vo)),
+                       "iterations_equal": bool(int(j["st"]["iterations"][0]) == int(so["iterations"][0])),
+                       "status_equal": bool(int(j["st"]["status"][0]) == int(so["status"][0]))})
+    base = {"value": N / (total / threads), "unit": "solves/s", "cores": threads, "kind": "port",
+            "extrapolated": True,
+            "sample": f"one instance of each of {len(run)} circuits (oracle FP32 path, one host thread each, "
+                      f"{wall:.1f} s wall), scaled by each circuit's instance count; chains of 600-4096 nodes "
+                      f"({len(jobs) - len(run)} circuits) extrapolated by n^3 per Newton iteration from the "
+                      f"{jref['nodes'] if jref else 0}-node chain",
+            "host": host_info(threads)}
+    log(f"mixed cpu baseline {base['value']:.2f} solves/s ({threads} threads, extrapolated)")
+    return base, parity
 
 
 def main():

@@ -274,15 +274,15 @@ enum { NR_PREC_FP64 = 0, NR_PREC_FP32_REFINE = 1 };
  * routed at solvers/circuitsim_solver.cpp:291-295).
  *  NR_LU_AUTO         multifrontal LU on the reference's Min-Fill ordering
  *                     (no pivoting), then each solve's relative residual
- *                     max|b - A x| / max|b| in FP64; a solve above 1e-8
- *                     (FP32 path: 0.1 on its float system), not finite or
- *                     with a pivot below 1e-25 is redone by the restated
- *                     solveLU_Pivoted (FP32 path: on the FP64 Jacobian and
- *                     the FP64 Newton residual).  Circuits above
+ *                     max|b - A x| / max|b| in FP64; a solve above 1e-8,
+ *                     not finite or with a pivot below 1e-25 is redone by
+ *                     the restated solveLU_Pivoted.  Circuits above
  *                     NR_DENSE_MAX_N nodes take the multifrontal solve
- *                     alone; there a DC solve or transient on the FP32
- *                     path takes a rejected iteration again on the FP64
- *                     path and finishes there (counted in lu_fallbacks).
+ *                     alone.  On the FP32 path a float solve above 0.1 (or
+ *                     not finite, or with a vanishing pivot) is not redone
+ *                     in float: the DC solve or transient takes that
+ *                     iteration again on the FP64 path from the same
+ *                     iterate and finishes there (counted in lu_fallbacks).
  *  NR_LU_MULTIFRONTAL the multifrontal LU alone.
  *  NR_LU_DENSE        the restated solveLU_Pivoted for every solve: the
  *                     reference's LU bit for bit, O(n^3) per Newton step;

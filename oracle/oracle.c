@@ -1318,9 +1318,10 @@ int orc_jacobian_f32(const nr_netlist* nl, const orc_topo* t, const double* v, d
  * the FP64 Newton residual rounded to float, vn = v + the float correction
  * (FP64 update).  The Newton residual b - A v of the reference's Norton
  * system is the KCL audit vector minus the gmin diagonal current.
- * With `promote` (DC, AUTO route, n > NR_DENSE_MAX_N) a float solve the
- * residual check rejects sets *fallback = 2 and leaves vn unset: the caller
- * takes the iteration again on the FP64 path and stays there. */
+ * With `promote` (the AUTO route) a float solve the residual check rejects
+ * sets *fallback = 2 and leaves vn unset: the caller takes the iteration
+ * again on the FP64 path (whose AUTO route has the dense fallback up to
+ * NR_DENSE_MAX_N nodes) and stays there. */
 static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_topo* t, const double* v,
                            double gmin, const tran_ctx* tc, int32_t lu_kind, int promote, double* vals, double* lin,
                            float* a32, float* r32, float* x32, double* res, double* vn, int32_t* fallback) {
@@ -1342,7 +1343,7 @@ static int newton_step_f32(const nr_netlist* nl, const orc_soa* s, const orc_top
             dense = rc != NR_OK || orc_lu_flag(t, ad, rd, vn, ORC_LU_ETA_FP32, NULL);
             *fallback = dense;
             free(ad); free(rd);
-            if (dense && t->n > NR_DENSE_MAX_N) {
+            if (dense && promote) {
                 *fallback = 2;
                 return NR_OK;
             }
@@ -1641,7 +1642,7 @@ int orc_solve_dc(const nr_netlist* nl, const orc_topo* t, const orc_dc_opts* opt
     double* vals = (double*)xmalloc(sizeof(double) * (size_t)t->n_stamps);
     double* av = (double*)xmalloc(sizeof(double) * (size_t)t->nnz_a);
     int p32 = (o.precision == ORC_PREC_FP32_REFINE);
-    const int promote = p32 && o.lu_kind == ORC_LU_AUTO && t->n > NR_DENSE_MAX_N;
+    const int promote = p32 && o.lu_kind == ORC_LU_AUTO;
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
@@ -1999,7 +2000,7 @@ int orc_solve_transient(const nr_netlist* nl, const orc_topo* t, const orc_tran_
     orc_arbiter arb;
     arb.prev = (double*)xmalloc(nb); arb.prev2 = (double*)xmalloc(nb); arb.dvdt = (double*)xmalloc(nb);
     int p32 = (o.precision == ORC_PREC_FP32_REFINE);
-    const int promote = p32 && o.lu_kind == ORC_LU_AUTO && t->n > NR_DENSE_MAX_N;
+    const int promote = p32 && o.lu_kind == ORC_LU_AUTO;
     float* a32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)t->nnz_a) : NULL;
     float* r32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;
     float* x32 = p32 ? (float*)xmalloc(sizeof(float) * (size_t)n) : NULL;

@@ -1,0 +1,86 @@
+"""Uneven sizes (the mixed workload, reduced): three circuit families at two
+sizes each, every circuit with its own sparsity pattern, Min-Fill ordering
+and Monte-Carlo instances, solved concurrently from three host threads on
+three contexts (one HIP stream each) on the FP32 + FP64-refinement path with
+hipGraph sweeps, as the benchmark runs them.
+
+- Every instance converges (KCL <= 1e-9 A): the chains through the linear
+  route's residual check and FP64 fallback.
+- Each circuit's concurrent results are bit-identical to the same circuit
+  solved alone afterwards (nothing leaks between streams).
+- Sampled instances equal the oracle's FP32 path: bit for bit on the MOSFET
+  circuits (no exp / log), within 1e-9 relative + 1e-12 V with equal
+  iteration counts on the diode meshes.
+"""
+from __future__ import annotations
+
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from karam_amd import _abi, netlists
+
+pytestmark = pytest.mark.gpu
+
+PREC = _abi.NR_PREC_FP32_REFINE
+
+
+def jobs():
+    mc = dict(sigma_vto=0.03, sigma_kp=0.03)
+    return [
+        ("mesh12", netlists.rd_mesh(12, seed=101), 24, {}),
+        ("mesh30", netlists.rd_mesh(30, seed=102), 12, {}),
+        ("chain100", netlists.inverter_chain(100), 8, mc),
+        ("chain300", netlists.inverter_chain(300), 4, mc),
+        ("logic500", netlists.random_logic(500, seed=201, width=32, p_inv=0.3, p_nand3=0.1), 32, mc),
+        ("logic2000", netlists.random_logic(2000, seed=202, width=64, p_inv=0.3, p_nand3=0.1), 16, mc),
+    ]
+
+
+def test_mixed_sizes_concurrently():
+    from karam_amd import nr
+    from oracle import oracle as O
+    J = jobs()
+    ctxs = [nr.Context(0) for _ in range(3)]
+    runs = []
+    for k, (name, nl, B, kw) in enumerate(J):
+        ctx = ctxs[k % 3]
+        T = nr.Topology(nl, ctx)
+        f = netlists.mc_fields(nl, B, seed=500 + k, **kw)
+        bt = nr.Batch(ctx, T, B)
+        bt.set_fields(f)
+        runs.append({"name": name, "nl": nl, "B": B, "f": f, "T": T, "bt": bt, "w": k % 3, "ctx": ctx})
+    print("mixed: " + ", ".join(f"{r['name']} n={r['T'].info['n']} B={r['B']}" for r in runs), flush=True)
+
+    def work(w):
+        for r in runs:
+            if r["w"] == w:
+                r["res"] = r["bt"].solve_dc(precision=PREC, use_graph=1)
+                r["V"], r["st"] = r["bt"].voltages(), r["bt"].stats()
+        return w
+
+    with ThreadPoolExecutor(max_workers=3) as ex:
+        list(ex.map(work, range(3)))
+    for r in runs:
+        st = r["st"]
+        ok = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
+        print(f"{r['name']}: {ok.sum()}/{r['B']} converged, iterations {st['iterations'].mean():.0f}", flush=True)
+        assert ok.all(), r["name"]
+        # the same circuit alone, afterwards
+        r["bt"].solve_dc(precision=PREC, use_graph=1)
+        np.testing.assert_array_equal(r["bt"].voltages(), r["V"], err_msg=r["name"])
+        np.testing.assert_array_equal(r["bt"].stats()["iterations"], st["iterations"], err_msg=r["name"])
+        # the oracle on two instances
+        sample = (0, r["B"] - 1)
+        To = O.Topology(r["nl"], ordering=r["T"].export(_abi.NR_EXPORT_ORDERING_RAW))
+        sub = {k: v[list(sample)] for k, v in r["f"].items()}
+        Vo, so = O.solve_dc_batch(r["nl"], To, sub, len(sample), threads=len(sample), precision=O.PREC_FP32_REFINE)
+        for j, b in enumerate(sample):
+            if r["name"].startswith("mesh"):
+                np.testing.assert_allclose(r["V"][b], Vo[j], rtol=1e-9, atol=1e-12, err_msg=r["name"])
+            else:
+                np.testing.assert_array_equal(r["V"][b], Vo[j], err_msg=r["name"])
+            assert st["iterations"][b] == so["iterations"][j], r["name"]
+            assert st["status"][b] == so["status"][j] and st["lu_fallbacks"][b] == so["lu_fallbacks"][j]
+        r["bt"].close()

@@ -60,8 +60,11 @@ def test_persistent_equals_level_launches(ctx, name, prec):
     bt.set_persistent(True, precision=prec)
     bt.solve_dc(precision=prec, record_timing=2)
     ks = bt.kernel_stats()
-    assert any(k.startswith("k_factor_persist") for k in ks), sorted(ks)
-    assert not any(k.startswith("k_factor_large") for k in ks), sorted(ks)
+    ty = "float>" if prec == _abi.NR_PREC_FP32_REFINE else "double>"
+    # (instances the FP32 path hands to FP64 run the FP64 kernels, whose
+    # launch mode is set separately)
+    assert any(k.startswith("k_factor_persist") and k.endswith(ty) for k in ks), sorted(ks)
+    assert not any(k.startswith("k_factor_large") and k.endswith(ty) for k in ks), sorted(ks)
     _same(ref, bt)
     # graph replay of the same sweeps
     bt.solve_dc(precision=prec, use_graph=1)
@@ -69,7 +72,7 @@ def test_persistent_equals_level_launches(ctx, name, prec):
     # and back to the per-level launches
     bt.set_persistent(False, precision=prec)
     bt.solve_dc(precision=prec, record_timing=2)
-    assert not any(k.startswith("k_factor_persist") for k in bt.kernel_stats())
+    assert not any(k.startswith("k_factor_persist") and k.endswith(ty) for k in bt.kernel_stats())
     _same(ref, bt)
 
 

@@ -779,6 +779,8 @@ def mixed_workload(a):
         if not a.no_tune:
             j["bt"].autotune(precision=prec, reps=2)
         tune_s += time.perf_counter() - t0
+        log(f"set up {fams[j['fam']]} n={j['T'].info['n']} B={j['cnt']}: symbolic {j['symbolic_s'] * 1e3:.0f} ms, "
+            f"tuned in {time.perf_counter() - t0:.1f} s")
 
     def run(w):
         for j in jobs:
@@ -787,6 +789,8 @@ def mixed_workload(a):
                 j["r"] = j["bt"].solve_dc(precision=prec, use_graph=1)
                 j["solve_s"] = time.perf_counter() - t0
                 j["st"] = j["bt"].stats()
+                log(f"solved {fams[j['fam']]} n={j['nodes']} B={j['cnt']} on stream {w}: "
+                    f"{j['solve_s'] * 1e3:.0f} ms")
         return w
 
     t0 = time.perf_counter()
@@ -797,6 +801,7 @@ def mixed_workload(a):
     kst = {}
     for j in jobs:
         j["bt"].solve_dc(precision=prec, record_timing=2)
+        log(f"kernel events {fams[j['fam']]} n={j['nodes']}")
         for k, x in j["bt"].kernel_stats().items():
             y = kst.setdefault(k, {"launches": 0, "ms": 0.0, "bytes": 0.0, "flops": 0.0})
             for q in y:
@@ -839,7 +844,10 @@ def mixed_workload(a):
         "value_with_symbolic": conv_total / (solve_s + symb_s),
         "circuit_solve_s_summed": sum(r_["solve_ms"] for r_ in rows) / 1e3,
         "families": per_fam, "circuits": rows,
-        "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst),
+        "roofline": roofline_either(kst),
+        "roofline_multifrontal": roofline({k: v for k, v in kst.items()
+                                           if k.startswith(("k_factor", "k_backsolve"))}, traffic=False),
+        "kernels": kernel_table(kst),
         "cpu_baseline": None,
     }
     if not a.no_cpu:
@@ -852,24 +860,21 @@ def mixed_cpu_baseline(a, jobs, prec):
     """The oracle on one instance of each circuit, each circuit on its own
     host thread (all at once), its FP32 path as the GPU ran; the workload's
     CPU time on C cores is sum(instances x seconds per instance) / C.  The
-    inverter chains above 600 nodes are not run: their Newton steps take the
-    dense O(n^3) fallback (up to 4096 nodes), so their time is extrapolated
-    from the largest chain run by (n / n_run)^3 per Newton iteration, with
-    the iterations the GPU took.  The sampled instances are also the parity
-    sample."""
+    sampled instances are also the parity sample."""
     from concurrent.futures import ThreadPoolExecutor
     from karam_amd import _abi
     from oracle import oracle as O
     threads = cpu_threads(a)
     oprec = O.PREC_FP32_REFINE if prec == _abi.NR_PREC_FP32_REFINE else O.PREC_FP64
-    dense_chain = lambda j: j["fam"] == 1 and 600 < j["nodes"] <= 4096  # noqa: E731
-    run = [j for j in jobs if not dense_chain(j)]
+    run = list(jobs)
 
     def one(j):
         To = O.Topology(j["nl"], ordering=j["T"].export(_abi.NR_EXPORT_ORDERING_RAW))
         t0 = time.perf_counter()
         Vo, so = O.solve_dc_batch(j["nl"], To, {k: v[:1] for k, v in j["f"].items()}, 1, threads=1, precision=oprec)
-        return time.perf_counter() - t0, Vo[0], so
+        dt = time.perf_counter() - t0
+        log(f"oracle n={j['nodes']}: {dt:.1f} s")
+        return dt, Vo[0], so
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=threads) as ex:
@@ -878,13 +883,6 @@ def mixed_cpu_baseline(a, jobs, prec):
     per = {}
     for j, (dt, Vo, so) in zip(run, res):
         per[id(j)] = dt
-    chains = [j for j in run if j["fam"] == 1 and j["nodes"] <= 600]
-    jref = max(chains, key=lambda j: j["nodes"]) if chains else None
-    for j in jobs:
-        if id(j) not in per and jref is not None:
-            it_ref = float(jref["st"]["iterations"][0])
-            it_j = float(j["st"]["iterations"].mean())
-            per[id(j)] = per[id(jref)] / max(it_ref, 1.0) * it_j * (j["nodes"] / jref["nodes"]) ** 3
     total = sum(j["cnt"] * per[id(j)] for j in jobs)
     N = sum(j["cnt"] for j in jobs)
     parity = []
@@ -893,14 +891,12 @@ def mixed_cpu_baseline(a, jobs, prec):
         err = float((np.abs(Vg - Vo) / (1e-9 * np.abs(Vo) + 1e-12)).max())
         parity.append({"family": ("rdmesh", "chain", "logic")[j["fam"]], "nodes": j["nodes"],
                        "oracle_s": dt, "err_over_tol": err, "bit_identical": bool(np.array_equal(Vg, Vo)),
-                       "iterations_equal": bool(int(j["st"]["iterations"][0]) == int(so["iterations"][0])),
+                       "iterations_gpu": int(j["st"]["iterations"][0]), "iterations_oracle": int(so["iterations"][0]),
                        "status_equal": bool(int(j["st"]["status"][0]) == int(so["status"][0]))})
     base = {"value": N / (total / threads), "unit": "solves/s", "cores": threads, "kind": "port",
             "extrapolated": True,
-            "sample": f"one instance of each of {len(run)} circuits (oracle FP32 path, one host thread each, "
-                      f"{wall:.1f} s wall), scaled by each circuit's instance count; chains of 600-4096 nodes "
-                      f"({len(jobs) - len(run)} circuits) extrapolated by n^3 per Newton iteration from the "
-                      f"{jref['nodes'] if jref else 0}-node chain",
+            "sample": f"instance 0 of each of the {len(run)} circuits (oracle FP32 path, AUTO route, one host "
+                      f"thread per circuit, {wall:.1f} s wall), each circuit's time scaled by its instance count",
             "host": host_info(threads)}
     log(f"mixed cpu baseline {base['value']:.2f} solves/s ({threads} threads, extrapolated)")
     return base, parity

@@ -808,7 +808,7 @@ static void supernodes(orc_topo* t) {
 /* a8: createCsr / buildPattern (math/linalg.h:281-431): sort the triplet
  * coordinates by (row, col), merge duplicates; the stamp map is each
  * triplet's merged slot (assembleFast's position_map lookup, :441-465). */
-static int64_t* g_sort_keys;
+static _Thread_local int64_t* g_sort_keys;   /* per thread: topologies may be built concurrently */
 static int cmp_key_idx(const void* a, const void* b) {
     int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
     int64_t kx = g_sort_keys[x], ky = g_sort_keys[y];

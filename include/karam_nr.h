@@ -301,7 +301,11 @@ typedef struct nr_solve_opts {
                                 (captured once per grid size and argument
                                 set; per-kernel timing is then not kept)   */
     int32_t fallback_plain;  /* 1: an instance whose seeded solve does not
-                                converge is solved again from the plain
+                                converge, or converges with a node beyond
+                                1000 x max(5 V, the largest V source) (the
+                                KCL audit has no gmin term, so a node only
+                                cut-off devices touch passes it at any
+                                voltage), is solved again from the plain
                                 GMIN-stepping start                           */
     int32_t trace_capacity;  /* > 0: keep each instance's last trace_capacity
                                 ConvergenceStep records (nr_batch_get_trace) */

@@ -713,11 +713,25 @@ __device__ void seed_initial_guess(const DevTopo& T, const DevBatch& P, int b, d
 // Plain-start fallback (one block per instance): an instance whose seeded
 // solve did not converge restarts from seedInitialGuess at the first GMIN
 // step; its iteration counts keep accumulating.
-__global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P) {
+// A seeded instance restarts from the plain start when its seeded solve did
+// not converge, or converged with a node beyond vbound: the KCL audit holds
+// no gmin term, so a node that only cut-off devices touch meets it at any
+// voltage, and a seed can leave one there (the plain start's GMIN stepping
+// does not).
+__global__ __launch_bounds__(256) void k_fallback(DevTopo T, DevBatch P, double vbound) {
+    __shared__ double red[4];
     const int b = blockIdx.x;
     InstState st = P.st[b];
-    if (st.status == NR_STATUS_CONVERGED) return;
     double* v = P.v + (size_t)b * T.n;
+    if (st.status == NR_STATUS_CONVERGED) {
+        double mx = 0.0;
+        for (int i = threadIdx.x; i < T.n; i += blockDim.x) {
+            const double a = v[i] < 0.0 ? -v[i] : v[i];
+            mx = (mx < a || a != a) ? a : mx;
+        }
+        mx = block_max(mx, red);
+        if (!st.seeded || mx <= vbound) return;
+    }
     seed_initial_guess(T, P, b, v);
     for (int i = threadIdx.x; i < T.n; i += blockDim.x) P.vI[IX(i, b, P.Bp)] = v[i];
     if (P.p32) residual_to_interleaved(T, P, b, v);

@@ -243,6 +243,9 @@ struct DevBatch {
     // path does not otherwise use
     double* res;
     int p32;
+    // per-instance MOSFET drain currents of the KCL audit in progress
+    // (instance-major [B][nM]; null: evaluated per KCL term)
+    double* mos_i;
     // DC diagnostics: per-instance state, the optional ring of the last
     // trace_cap records per instance, the final per-node statistics
     DiagState* diag;
@@ -435,7 +438,10 @@ constexpr int HUB_CHUNK = 1024;
 
 // One KCL term (calculatePhysicalResiduals, physics/circuitsim.h:1250-1671,
 // MOSFET term restated with SoA parameters), with its sign applied
-__device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const double* v, int t) {
+// mi (optional): the instance's MOSFET drain currents at v, evaluated once
+// per audit (block_mos_currents); a MOSFET term then reads its current there
+__device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const double* v, int t,
+                               const double* mi = nullptr) {
 #pragma clang fp contract(off)
     const int n = T.n;
     const int idx = term_idx(t);
@@ -470,6 +476,10 @@ __device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const
         break;
     }
     default: {  // TK_MOS
+        if (mi) {
+            val = mi[idx];
+            break;
+        }
         double vgs, vds, ids, gm, gds;
         mos_eval(volt(v, n, T.m_g[idx]), volt(v, n, T.m_d[idx]), volt(v, n, T.m_s[idx]), T.m_w[idx],
                  T.m_l[idx], P.kp[(size_t)b * T.nM + idx], P.vth[(size_t)b * T.nM + idx], T.m_lam[idx],
@@ -482,11 +492,27 @@ __device__ double kcl_term_val(const DevTopo& T, const DevBatch& P, int b, const
 }
 
 // KCL residual of one node: its terms summed in list order
-__device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const double* v, int i) {
+__device__ double kcl_node(const DevTopo& T, const DevBatch& P, int b, const double* v, int i,
+                           const double* mi = nullptr) {
 #pragma clang fp contract(off)
     double s = 0.0;
-    for (int k = T.kcl_ptr[i]; k < T.kcl_ptr[i + 1]; ++k) s = s + kcl_term_val(T, P, b, v, T.kcl_term[k]);
+    for (int k = T.kcl_ptr[i]; k < T.kcl_ptr[i + 1]; ++k) s = s + kcl_term_val(T, P, b, v, T.kcl_term[k], mi);
     return s;
+}
+
+// the MOSFET drain currents of instance b at v, one evaluation per device
+// (the KCL list holds two terms per MOSFET), into mi; whole block
+__device__ void block_mos_currents(const DevTopo& T, const DevBatch& P, int b, const double* v, double* mi) {
+#pragma clang fp contract(off)
+    const int n = T.n;
+    for (int k = threadIdx.x; k < T.nM; k += blockDim.x) {
+        double vgs, vds, ids, gm, gds;
+        mos_eval(volt(v, n, T.m_g[k]), volt(v, n, T.m_d[k]), volt(v, n, T.m_s[k]), T.m_w[k], T.m_l[k],
+                 P.kp[(size_t)b * T.nM + k], P.vth[(size_t)b * T.nM + k], T.m_lam[k], T.m_pmos[k], vgs, vds, ids,
+                 gm, gds);
+        mi[k] = ids;
+    }
+    __syncthreads();
 }
 
 // max |KCL| over the nodes of instance b (whole block); put(i, r) receives
@@ -497,10 +523,12 @@ template <typename Put>
 __device__ double block_kcl_put(const DevTopo& T, const DevBatch& P, int b, const double* v, double* red, Put put) {
 #pragma clang fp contract(off)
     __shared__ double hbuf[HUB_CHUNK];
+    double* mi = (P.mos_i && T.nM > 0) ? P.mos_i + (size_t)b * T.nM : nullptr;
+    if (mi) block_mos_currents(T, P, b, v, mi);
     double mx = 0.0;
     for (int i = threadIdx.x; i < T.n; i += blockDim.x) {
         if (T.kcl_ptr[i + 1] - T.kcl_ptr[i] > KCL_HUB) continue;
-        const double r = kcl_node(T, P, b, v, i);
+        const double r = kcl_node(T, P, b, v, i, mi);
         put(i, r);
         mx = ref_max(mx, fabs(r));
     }
@@ -511,7 +539,8 @@ __device__ double block_kcl_put(const DevTopo& T, const DevBatch& P, int b, cons
         for (int base = k0; base < k1; base += HUB_CHUNK) {
             const int cnt = min(HUB_CHUNK, k1 - base);
             __syncthreads();
-            for (int q = threadIdx.x; q < cnt; q += blockDim.x) hbuf[q] = kcl_term_val(T, P, b, v, T.kcl_term[base + q]);
+            for (int q = threadIdx.x; q < cnt; q += blockDim.x)
+                hbuf[q] = kcl_term_val(T, P, b, v, T.kcl_term[base + q], mi);
             __syncthreads();
             if (threadIdx.x == 0)
                 for (int q = 0; q < cnt; ++q) s = s + hbuf[q];

@@ -64,6 +64,30 @@ def copy_probe(gib=2):
     print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}), flush=True)
 
 
+def seeder_probe():
+    """The seeder's products at the learned-seeder configuration's shape,
+    for profiling: an MLP 512 -> 3 x 1024 -> 20,000, 8192 synthetic
+    training rows, two epochs (batch 1024), then three predictions of the
+    8192 rows."""
+    from karam_amd import nr
+    ctx = nr.Context(0)
+    rng = np.random.default_rng(0)
+    F, O, rows = SEEDER_FEATURES, 20000, 8192
+    s = nr.Seeder(ctx, F, O, hidden=SEEDER_HIDDEN, n_hidden=SEEDER_LAYERS)
+    X = rng.standard_normal((rows, F)).astype(np.float32)
+    s.record_solution(X, rng.standard_normal((rows, O)) * 0.1 + 0.9)
+    t0 = time.perf_counter()
+    loss = s.train(epochs=2, batch=1024)
+    t_train = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(3):
+        s.predict_seed(X)
+    t_pred = (time.perf_counter() - t0) / 3
+    print(json.dumps({"seeder_probe": {"train_2_epochs_ms": t_train * 1e3, "loss": loss,
+                                       "predict_8192_ms_incl_host_copies": t_pred * 1e3}}), flush=True)
+    return 0
+
+
 def load_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (lines `traffic_bytes_per_launch | <kernel> | <bytes>`), or None."""
@@ -1116,6 +1140,7 @@ def main():
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--seeder-probe", action="store_true", help="the seeder's products at the learned-seeder shape")
     ap.add_argument("--no-tune", action="store_true", help="default launch classes (no table, no tuning)")
     ap.add_argument("--retune", action="store_true", help="run the autotuner and write its table to --tune-table")
     ap.add_argument("--tune-table", default=os.path.join(ROOT, "karam_amd", "tuning", "rdmesh64_b4096.json"),
@@ -1123,6 +1148,8 @@ def main():
     a = ap.parse_args()
     if a.copy_probe:
         return copy_probe()
+    if a.seeder_probe:
+        return seeder_probe()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: Monte-Carlo instances shard by contiguous ranges with no exchange, "
               "and multi-GPU runs are not part of this build (use --gpus 1)", file=sys.stderr)

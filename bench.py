@@ -311,14 +311,14 @@ def seeded_path(a, ctx, nl, T, bt, fields, B, V64):
 
     def leg(prec, set_seed, label):
         set_seed()
-        bt.solve_dc(seeded_start_step=18, precision=prec, polish=1)         # warm-up
+        bt.solve_dc(seeded_start_step=18, precision=prec, polish=1, use_graph=a.graph)         # warm-up
         t_set, sweeps = 0.0, 0
         t0 = time.perf_counter()
         for k in range(a.steps):
             tp = time.perf_counter()
             set_seed()
             t_set += time.perf_counter() - tp
-            r = bt.solve_dc(seeded_start_step=18, precision=prec, polish=1)
+            r = bt.solve_dc(seeded_start_step=18, precision=prec, polish=1, use_graph=a.graph)
             st = bt.stats()
             sweeps = r["batched_iterations"]
             log(f"{label} step {k}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}, sweeps {sweeps}")
@@ -1150,6 +1150,7 @@ def main():
     ap.add_argument("--hightw-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
+    ap.add_argument("--graph", type=int, default=1, help="mesh workload: replay DC sweeps from hipGraphs")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--seeder-probe", action="store_true", help="the seeder's products at the learned-seeder shape")
     ap.add_argument("--no-tune", action="store_true", help="default launch classes (no table, no tuning)")
@@ -1194,7 +1195,7 @@ def main():
     tune = {}
     precs = (("fp64", _abi.NR_PREC_FP64), ("fp32_refine", _abi.NR_PREC_FP32_REFINE))
     if a.retune:                      # untimed: launch shapes per front-order bucket, per precision
-        table, persistent = {}, {}
+        table, persistent, fanout = {}, {}, {}
         for key, prec in precs:
             t0 = time.perf_counter()
             tu = bt.autotune(precision=prec, reps=3)
@@ -1205,31 +1206,37 @@ def main():
                                      for b in tu["buckets"]]}
             table[key] = [[b["team"], b["block"], b["full"]] for b in tu["buckets"]]
             persistent[key] = int(tu["persistent"])
+            fanout[key] = int(tu["fanout"])
             tune[key].update({"factor_solve_ms_levels": tu["ms_levels"],
-                              "factor_solve_ms_persistent": tu["ms_persistent"], "persistent": tu["persistent"]})
+                              "factor_solve_ms_persistent": tu["ms_persistent"], "persistent": tu["persistent"],
+                              "factor_solve_ms_graph": tu["ms_graph"],
+                              "factor_solve_ms_graph_fanout": tu["ms_graph_fanout"], "fanout": tu["fanout"]})
             log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution "
-                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f})")
+                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f}; graph "
+                f"{tu['ms_graph']:.2f}, fanned out {tu['ms_graph_fanout']:.2f})")
         with open(a.tune_table, "w") as fo:
-            json.dump({"workload": f"rdmesh{a.side}, {B} instances", "classes": table, "persistent": persistent},
-                      fo, indent=1)
+            json.dump({"workload": f"rdmesh{a.side}, {B} instances", "classes": table, "persistent": persistent,
+                       "fanout": fanout}, fo, indent=1)
         tune["table_written"] = os.path.relpath(a.tune_table, ROOT)
     elif not a.no_tune and os.path.exists(a.tune_table):
         # the committed table from a tuning run on MI355X: every run (and every
         # profile of it) uses the same launch class per front-order bucket
         saved = json.load(open(a.tune_table))
-        table, persistent = saved["classes"], saved.get("persistent", {})
+        table, persistent, fanout = saved["classes"], saved.get("persistent", {}), saved.get("fanout", {})
         for key, prec in precs:
             for bucket, (team, block, full) in enumerate(table[key]):
                 bt.set_launch_class(bucket, team, block, full=full, precision=prec)
             bt.set_persistent(bool(persistent.get(key, 0)), precision=prec)
-            tune[key] = {"buckets": table[key], "persistent": int(persistent.get(key, 0))}
+            bt.set_fanout(bool(fanout.get(key, 0)), precision=prec)
+            tune[key] = {"buckets": table[key], "persistent": int(persistent.get(key, 0)),
+                         "fanout": int(fanout.get(key, 0))}
         tune["table"] = os.path.relpath(a.tune_table, ROOT)
         log(f"launch classes from {tune['table']}")
     for w in range(a.warmup):
-        r = bt.solve_dc()
+        r = bt.solve_dc(use_graph=a.graph)
         log(f"warmup {w}: {r['solve_ms']:.1f} ms, converged {r['n_converged']}")
 
-    rate, ms, kst, st, r = timed_solves(bt, a.steps)
+    rate, ms, kst, st, r = timed_solves(bt, a.steps, use_graph=a.graph)
     V = bt.voltages()
     evals = r["device_evals"]
     it = st["iterations"]
@@ -1242,7 +1249,7 @@ def main():
                    "instances": B, "nodes": T.info["n"], "resistors": len(nl.res_ohms),
                    "diodes": len(nl.dio_is), "current_sources": len(nl.isrc_amps),
                    "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
-                   "gmin_steps": 19, "symbolic_ms": symbolic_ms,
+                   "gmin_steps": 19, "symbolic_ms": symbolic_ms, "sweeps_from_hip_graphs": bool(a.graph),
                    "batched_sweeps_per_step": r["batched_iterations"]},
         "nr_iters_per_solve": {"plain": iter_stats(it), "plain_fp32_refine": None, "seeded": None,
                                "seeded_fp32_refine": None},
@@ -1253,8 +1260,9 @@ def main():
         "autotune": tune or None,
     }
     # the FP32 path on the same instances, its error against the FP64 results
-    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)                # warm-up
-    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE)
+    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE, use_graph=a.graph)                # warm-up
+    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE,
+                                                  use_graph=a.graph)
     V32 = bt.voltages()
     out["nr_iters_per_solve"]["plain_fp32_refine"] = iter_stats(st32["iterations"])
     out["fp32_refine"] = {"value": rate32, "unit": "solves/s", "ms_per_step": ms32,

@@ -401,10 +401,14 @@ NR_API void nr_transient_opts_default(nr_transient_opts* opts);
  * parameters; factorisation + back substitution of every instance, `reps`
  * times per candidate, one bucket after another).  The choice is stored in
  * the batch's topology, for the given precision, and used by every later
- * solve of batches of that topology.  It does not change any result: the
- * elimination performs the same operations in the same order whatever the
- * team size.  Overwrites the batch's solution state; not to be run while a
- * batch of the same topology is solving. */
+ * solve of batches of that topology.  It then times the whole-phase kernels
+ * (nr_batch_set_persistent) against the tuned per-level launches, and, when
+ * those stay, the per-level launches replayed from a hipGraph on one stream
+ * against the level fan-out (nr_batch_set_fanout), keeping the faster each
+ * time.  It does not change any result: the elimination performs the same
+ * operations in the same order whatever the team size and launch mode.
+ * Overwrites the batch's solution state; not to be run while a batch of the
+ * same topology is solving. */
 enum { NR_TUNE_BUCKETS = 8 };
 typedef struct nr_autotune_result {
     int32_t n_buckets;
@@ -417,6 +421,9 @@ typedef struct nr_autotune_result {
     double ms_levels;                      /* the same with the tuned per-level launches */
     double ms_persistent;                  /* the same with the whole-phase kernels (-1: not offered) */
     int32_t persistent;                    /* 1: the whole-phase kernels were kept (faster) */
+    double ms_graph;                       /* per-level launches replayed from a hipGraph, one stream (-1: not timed) */
+    double ms_graph_fanout;                /* the same with each level's launches on several streams */
+    int32_t fanout;                        /* 1: the level fan-out was kept (faster) */
 } nr_autotune_result;
 NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
                              nr_autotune_result* out);
@@ -441,6 +448,18 @@ NR_API int nr_batch_set_launch_class(nr_batch* batch, int32_t precision, int32_t
  * the back substitution's LDS.  nr_batch_autotune also times it against
  * the tuned per-level launches and keeps the faster. */
 NR_API int nr_batch_set_persistent(nr_batch* batch, int32_t precision, int32_t on);
+/* Per-level launches of one elimination-tree level on several streams: the
+ * team classes' launches of a level (and the small-front classes' launches
+ * of a small-front level) go to the context's stream and up to three side
+ * streams that fork from it before the level and join it after, so that a
+ * level's classes run concurrently; the levels stay in order.  Part of the
+ * per-level scheduling of solveSparse_LU's numeric phase
+ * (math/sparse_lu.cpp:134-330); no result changes (the fronts of one level
+ * are independent).  Off while per-kernel events are recorded
+ * (record_timing = 2).  Kept by the topology, FP64 and FP32 separately;
+ * AC and noise keep one stream.  nr_batch_autotune times it and keeps the
+ * faster. */
+NR_API int nr_batch_set_fanout(nr_batch* batch, int32_t precision, int32_t on);
 
 /* Transient of every instance (the restated stepTransient): capacitor
  * companions of the instance's current method (trapezoidal G = 2C/h with

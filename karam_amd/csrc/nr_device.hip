@@ -47,6 +47,7 @@
 #include <random>
 #include <string>
 #include <utility>
+#include <utility>
 #include <vector>
 
 #include "karam_nr.h"
@@ -60,10 +61,19 @@ using karam::Topology;
 // ---------------------------------------------------------------------------
 static thread_local std::string g_lib_error;
 
+// concurrent launches of one elimination-tree level: the context's stream
+// and FAN_WAYS - 1 side streams
+constexpr int FAN_WAYS = 4;
+
 struct nr_ctx {
     int device = 0;
     int verbose = 0;
     hipStream_t stream = nullptr;
+    // side streams that fork from `stream` and join it again around the
+    // launches of one level (default priority, created with the context)
+    hipStream_t fan[FAN_WAYS - 1] = {};
+    hipEvent_t fan_fork = nullptr;
+    hipEvent_t fan_join[FAN_WAYS - 1] = {};
     std::string err;
 };
 
@@ -2138,6 +2148,11 @@ constexpr double LU_ETA_FP64 = 1e-8;     // bound of the FP64 path's ratio
 // there); solves the missing pivoting spoils sit at 1e16 and above.
 constexpr double LU_ETA_FP32 = 0.1;
 constexpr int DENSE_THREADS = 1024;
+// blocked dense LU: widest panel (pivot steps per pass over the trailing
+// matrix) and the dynamic LDS of k_dense (the rhs, then the panel; static
+// LDS stays below 2 KB of the 160 KB a workgroup may hold)
+constexpr int DENSE_NB_MAX = 16;
+constexpr size_t DENSE_LDS = 160 * 1024 - 2048;
 
 // NR_LU_AUTO, after the back substitution: per instance, max_i |b_i - (A x)_i|
 // and max_i |b_i| from the assembled values (the factorisation leaves the A
@@ -2253,13 +2268,26 @@ __device__ __forceinline__ void piv_better(double& v, int& k, double v2, int k2)
 // columns to each rhs entry in increasing order, and the backward
 // substitution subtracts each row's products in increasing column order
 // before dividing by the diagonal.  So the solution is bit-identical to the
-// sequential routine.  It is written to xm in the layout k_update reads
+// sequential routine.
+// The elimination is blocked without changing any element's operations: a
+// panel of w columns (w <= DENSE_NB_MAX, as many as the LDS holds for the
+// remaining rows) is factored in LDS step by step (pivot search, the swap
+// of the panel part of the two rows and of the rhs, multipliers, the
+// panel's own updates); then, one thread per column outside the panel, the
+// panel's row swaps are applied in step order to the rest of each row, each
+// pivot row's entries right of the panel take the earlier pivot rows'
+// updates in step order, and every trailing entry takes the panel's w
+// updates in step order, one pass over the trailing matrix per panel
+// instead of one per step.  An entry's value depends only on the updates it
+// receives and their order, and a row's multipliers move with it, so the
+// results are those of the unblocked routine bit for bit.
+// The solution is written to xm in the layout k_update reads
 // (permuted rows).  On the FP32 path the system is the FP64 one: every A
 // entry summed as k_assemble sums it (linear part, FP64 device conductances
 // in term order, the gmin diagonal) and the FP64 Newton residual, for the
 // correction; rounding an ill-conditioned Jacobian to float alone moves its
 // Newton step.  count_fb: add one to the instance's fallback count.
-// Dynamic LDS: 3 n doubles.
+// Dynamic LDS: DENSE_LDS bytes (n <= 4096).
 __device__ double slot_value64(const DevTopo& T, const DevBatch& P, int b, int q, double gmin) {
 #pragma clang fp contract(off)
     const int n = T.n, Bp = P.Bp;
@@ -2293,11 +2321,14 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense(DevTopo T, DevBatch P, 
     __shared__ double s_v;
     __shared__ double rv[DENSE_THREADS / 64];
     __shared__ int rk[DENSE_THREADS / 64];
+    __shared__ int spiv[DENSE_NB_MAX];
     const int n = T.n, tid = threadIdx.x, nt = blockDim.x, Bp = P.Bp;
     const int lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
-    double* srow = dsm;            // pivot row (factor); products (backward)
-    double* sl = dsm + n;          // multipliers (factor); x (backward)
-    double* sy = dsm + 2 * n;      // rhs -> y
+    double* sy = dsm;              // rhs -> y
+    double* pan = dsm + n;         // panel (factor); products (backward)
+    double* srow = pan;
+    double* sl = pan + n;          // x (backward)
+    const int cap = (int)(DENSE_LDS / sizeof(double)) - n;     // panel values
     double* M = P.dn_mat + (size_t)blockIdx.x * (size_t)n * (size_t)n;
     const V* pool = vp<V>(P.pool);
     const V* Al = vp<V>(P.Al);
@@ -2332,48 +2363,136 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense(DevTopo T, DevBatch P, 
         }
         __syncthreads();
         int sing = 0;
-        for (int i = 0; i < n; ++i) {
-            double bv = 0.0;
-            int bk = 0x7fffffff;
-            for (int k = i + tid; k < n; k += nt) {
-                const double v = fabs(M[(size_t)k * n + i]);
-                if (v > bv) { bv = v; bk = k; }
-            }
-            for (int o = 32; o > 0; o >>= 1) piv_better(bv, bk, __shfl_xor(bv, o), __shfl_xor(bk, o));
-            if (lane == 0) { rv[wv] = bv; rk[wv] = bk; }
-            __syncthreads();
-            if (tid == 0) {
-                double v = rv[0];
-                int k = rk[0];
-                for (int u = 1; u < nw; ++u) piv_better(v, k, rv[u], rk[u]);
-                s_v = v;
-                s_k = v > 0.0 ? k : i;
+        for (int i0 = 0, w = 0; i0 < n && !sing; i0 += w) {
+            const int rows = n - i0;
+            w = min(DENSE_NB_MAX, min(rows, cap / rows));
+            // the panel: rows i0.., columns i0..i0+w-1, row-major (stride w)
+            for (int e = tid; e < rows * w; e += nt) {
+                const int r = e / w, k = e - r * w;
+                pan[e] = M[(size_t)(i0 + r) * n + i0 + k];
             }
             __syncthreads();
-            const int p = s_k;
-            if (s_v < 1e-25) { sing = 1; break; }
-            if (p != i) {
-                for (int c = tid; c < n; c += nt) {
-                    const double tmp = M[(size_t)i * n + c];
-                    M[(size_t)i * n + c] = M[(size_t)p * n + c];
-                    M[(size_t)p * n + c] = tmp;
+            for (int k = 0; k < w; ++k) {
+                double bv = 0.0;
+                int bk = 0x7fffffff;
+                for (int r = k + tid; r < rows; r += nt) {
+                    const double v = fabs(pan[r * w + k]);
+                    if (v > bv) { bv = v; bk = r; }
                 }
-                if (tid == 0) { const double tb = sy[i]; sy[i] = sy[p]; sy[p] = tb; }
+                for (int o = 32; o > 0; o >>= 1) piv_better(bv, bk, __shfl_xor(bv, o), __shfl_xor(bk, o));
+                if (lane == 0) { rv[wv] = bv; rk[wv] = bk; }
                 __syncthreads();
+                if (tid == 0) {
+                    double v = rv[0];
+                    int kk = rk[0];
+                    for (int u = 1; u < nw; ++u) piv_better(v, kk, rv[u], rk[u]);
+                    s_v = v;
+                    s_k = v > 0.0 ? kk : k;
+                    spiv[k] = i0 + s_k;
+                }
+                __syncthreads();
+                if (s_v < 1e-25) { sing = 1; break; }
+                const int p = s_k;
+                if (p != k) {
+                    for (int c = tid; c < w; c += nt) {
+                        const double tmp = pan[k * w + c];
+                        pan[k * w + c] = pan[p * w + c];
+                        pan[p * w + c] = tmp;
+                    }
+                    if (tid == 0) { const double tb = sy[i0 + k]; sy[i0 + k] = sy[i0 + p]; sy[i0 + p] = tb; }
+                    __syncthreads();
+                }
+                const double piv = pan[k * w + k];
+                for (int r = k + 1 + tid; r < rows; r += nt) pan[r * w + k] = pan[r * w + k] / piv;
+                __syncthreads();
+                if (k + 1 < w) {
+                    for (int r = k + 1 + tid; r < rows; r += nt) {
+                        const double l = pan[r * w + k];
+                        if (l != 0.0)
+                            for (int c = k + 1; c < w; ++c) pan[r * w + c] = pan[r * w + c] - l * pan[k * w + c];
+                    }
+                    __syncthreads();
+                }
             }
-            const double piv = M[(size_t)i * n + i];
-            for (int c = i + 1 + tid; c < n; c += nt) srow[c] = M[(size_t)i * n + c];
-            for (int j = i + 1 + tid; j < n; j += nt) {
-                const double l = M[(size_t)j * n + i] / piv;
-                M[(size_t)j * n + i] = l;
-                sl[j] = l;
+            if (sing) break;
+            for (int e = tid; e < rows * w; e += nt) {
+                const int r = e / w, k = e - r * w;
+                M[(size_t)(i0 + r) * n + i0 + k] = pan[e];
+            }
+            // the panel's row swaps on every other column, in step order
+            for (int cc = tid; cc < n - w; cc += nt) {
+                const int c = cc < i0 ? cc : cc + w;
+                for (int k = 0; k < w; ++k) {
+                    const int q = spiv[k];
+                    if (q != i0 + k) {
+                        const double a = M[(size_t)(i0 + k) * n + c], bq = M[(size_t)q * n + c];
+                        M[(size_t)(i0 + k) * n + c] = bq;
+                        M[(size_t)q * n + c] = a;
+                    }
+                }
             }
             __syncthreads();
-            for (int j = i + 1 + wv; j < n; j += nw) {
-                const double l = sl[j];
-                if (l != 0.0) {
-                    double* rj = M + (size_t)j * n;
-                    for (int c = i + 1 + lane; c < n; c += 64) rj[c] = rj[c] - l * srow[c];
+            // right of the panel: the pivot rows (U12), then the trailing
+            // rows; a column's trailing rows may be split over S segments
+            // (each recomputes the column's U12 in registers, segment 0
+            // stores it)
+            const int c0 = i0 + w, nh = n - c0, nr = rows - w;
+            if (nh > 0) {
+                const int S = max(1, nt / nh);
+                const int len = (nr + S - 1) / S;
+                for (int it = tid; it < nh * S; it += nt) {
+                    const int c = c0 + it % nh, seg = it / nh;
+                    double u[DENSE_NB_MAX];
+#pragma unroll
+                    for (int k = 0; k < DENSE_NB_MAX; ++k) u[k] = k < w ? M[(size_t)(i0 + k) * n + c] : 0.0;
+#pragma unroll
+                    for (int k = 1; k < DENSE_NB_MAX; ++k) {
+                        if (k < w) {
+                            double a = u[k];
+#pragma unroll
+                            for (int q = 0; q < k; ++q) {
+                                const double l = pan[k * w + q];
+                                if (l != 0.0) a = a - l * u[q];
+                            }
+                            u[k] = a;
+                        }
+                    }
+                    if (seg == 0)
+#pragma unroll
+                        for (int k = 1; k < DENSE_NB_MAX; ++k)
+                            if (k < w) M[(size_t)(i0 + k) * n + c] = u[k];
+                    const int r0 = w + seg * len, r1 = min(rows, w + (seg + 1) * len);
+                    int r = r0;
+                    for (; r + 3 < r1; r += 4) {
+                        double a[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) a[t] = M[(size_t)(i0 + r + t) * n + c];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const double* lr = pan + (r + t) * w;
+#pragma unroll
+                            for (int k = 0; k < DENSE_NB_MAX; ++k) {
+                                if (k < w) {
+                                    const double l = lr[k];
+                                    if (l != 0.0) a[t] = a[t] - l * u[k];
+                                }
+                            }
+                        }
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) M[(size_t)(i0 + r + t) * n + c] = a[t];
+                    }
+                    for (; r < r1; ++r) {
+                        const double* lr = pan + r * w;
+                        double a = M[(size_t)(i0 + r) * n + c];
+#pragma unroll
+                        for (int k = 0; k < DENSE_NB_MAX; ++k) {
+                            if (k < w) {
+                                const double l = lr[k];
+                                if (l != 0.0) a = a - l * u[k];
+                            }
+                        }
+                        M[(size_t)(i0 + r) * n + c] = a;
+                    }
                 }
             }
             __syncthreads();

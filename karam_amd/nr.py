@@ -48,6 +48,7 @@ EXPORTED = [
     "nr_spice_element_name", "nr_spice_warning", "nr_spice_vsrc_ac",
     "nr_ac_opts_default", "nr_batch_ac", "nr_batch_get_ac", "nr_batch_get_ac_freqs", "nr_batch_get_ac_status",
     "nr_batch_get_diagnostics", "nr_batch_get_trace", "nr_batch_set_launch_class", "nr_batch_set_persistent",
+    "nr_batch_set_fanout",
     "nr_noise_opts_default", "nr_batch_noise", "nr_batch_get_noise", "nr_batch_get_noise_vrms",
     "nr_batch_get_noise_freqs", "nr_batch_get_noise_status",
 ]
@@ -85,6 +86,7 @@ def lib():
         L.nr_batch_get_diagnostics.argtypes = [_vp, C.POINTER(_abi.NrDiagnostics), C.c_int64]
         L.nr_batch_set_launch_class.argtypes = [_vp] + [C.c_int32] * 5
         L.nr_batch_set_persistent.argtypes = [_vp, C.c_int32, C.c_int32]
+        L.nr_batch_set_fanout.argtypes = [_vp, C.c_int32, C.c_int32]
         L.nr_batch_get_trace.argtypes = [_vp, C.POINTER(_abi.NrConvStep), C.c_int64]
         L.nr_noise_opts_default.argtypes = [C.POINTER(_abi.NrNoiseOpts)]
         L.nr_noise_opts_default.restype = None
@@ -431,6 +433,11 @@ class Batch:
         the back substitution (nr_batch_set_persistent); kept by the topology."""
         _check(lib().nr_batch_set_persistent(self.h, int(precision), int(bool(on))), self.ctx)
 
+    def set_fanout(self, on=True, precision=_abi.NR_PREC_FP64):
+        """The launches of one elimination-tree level on concurrent streams
+        (nr_batch_set_fanout); kept by the topology."""
+        _check(lib().nr_batch_set_fanout(self.h, int(precision), int(bool(on))), self.ctx)
+
     def autotune(self, precision=_abi.NR_PREC_FP64, reps=3):
         """Choose the large-front launch shape per front-order bucket by timing
         the candidates on this batch (nr_batch_autotune); the choice is kept
@@ -441,7 +448,8 @@ class Batch:
         return {"buckets": [{"rmax": r.bucket_rmax[b], "team": r.team[b], "block": r.block[b], "fronts": r.fronts[b],
                              "full": r.full[b]} for b in range(r.n_buckets)],
                 "ms_before": r.ms_before, "ms_after": r.ms_after, "ms_levels": r.ms_levels,
-                "ms_persistent": r.ms_persistent, "persistent": r.persistent}
+                "ms_persistent": r.ms_persistent, "persistent": r.persistent, "ms_graph": r.ms_graph,
+                "ms_graph_fanout": r.ms_graph_fanout, "fanout": r.fanout}
 
     def waveform(self):
         """Probe voltages of the last transient's waveform ring,

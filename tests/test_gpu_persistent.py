@@ -1,5 +1,6 @@
 """The whole-phase large-front kernels (k_factor_persist, k_backsolve_persist,
-nr_batch_set_persistent) against the per-level launches: one block per
+nr_batch_set_persistent) and the level fan-out (nr_batch_set_fanout)
+against the per-level launches on one stream: one block per
 instance walks every elimination-tree level of the large fronts, so the
 solves must give the same voltages, iteration counts, statuses and KCL bit
 for bit, in FP64 and on the FP32 path, eagerly and replayed from hipGraphs,
@@ -76,6 +77,27 @@ def test_persistent_equals_level_launches(ctx, name, prec):
     _same(ref, bt)
 
 
+@pytest.mark.parametrize("prec", [_abi.NR_PREC_FP64, _abi.NR_PREC_FP32_REFINE])
+@pytest.mark.parametrize("name", ["rdmesh64", "logic2000"])
+def test_fanout_equals_one_stream(ctx, name, prec):
+    """Each level's launches on several streams (nr_batch_set_fanout), eager
+    and replayed from hipGraphs, give the one-stream results bit for bit."""
+    nl, B, kw = circuits()[name]
+    f = netlists.mc_fields(nl, B, seed=45, **kw)
+    ref, bt = _pair(ctx, nl, B, f)
+    print(f"fanout {name} prec={prec} B={B}", flush=True)
+    ref.solve_dc(precision=prec)
+    bt.set_fanout(True, precision=prec)
+    bt.solve_dc(precision=prec)
+    _same(ref, bt)
+    for _ in range(2):
+        bt.solve_dc(precision=prec, use_graph=1)
+        _same(ref, bt)
+    bt.set_fanout(False, precision=prec)
+    bt.solve_dc(precision=prec, use_graph=1)
+    _same(ref, bt)
+
+
 def test_persistent_transient(ctx):
     nl, B, kw = circuits()["logic2000"]
     B = 16
@@ -102,6 +124,11 @@ def test_autotune_times_the_persistent_kernels(ctx):
     tune = bt.autotune(reps=2)
     assert tune["ms_levels"] > 0 and tune["ms_persistent"] > 0
     assert tune["ms_after"] == min(tune["ms_levels"], tune["ms_persistent"]) or tune["persistent"] == 0
+    if not tune["persistent"]:
+        assert tune["ms_graph"] > 0 and tune["ms_graph_fanout"] > 0
+        assert tune["fanout"] == int(tune["ms_graph_fanout"] < 0.99 * tune["ms_graph"])
+    bt.solve_dc(use_graph=1)
+    _same(ref, bt)
     bt.solve_dc()
     _same(ref, bt)
 

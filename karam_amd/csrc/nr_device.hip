@@ -2152,6 +2152,7 @@ constexpr int DENSE_THREADS = 1024;
 // matrix) and the dynamic LDS of k_dense (the rhs, then the panel; static
 // LDS stays below 2 KB of the 160 KB a workgroup may hold)
 constexpr int DENSE_NB_MAX = 16;
+constexpr int DENSE_ROWS = 4;      // trailing rows per lane in flight
 constexpr size_t DENSE_LDS = 160 * 1024 - 2048;
 
 // NR_LU_AUTO, after the back substitution: per instance, max_i |b_i - (A x)_i|
@@ -2463,6 +2464,30 @@ __global__ __launch_bounds__(DENSE_THREADS) void k_dense(DevTopo T, DevBatch P, 
                             if (k < w) M[(size_t)(i0 + k) * n + c] = u[k];
                     const int r0 = w + seg * len, r1 = min(rows, w + (seg + 1) * len);
                     int r = r0;
+                    // with every u finite, a - l * u equals a when l is zero
+                    // (the matrix holds no -0: entries start as 0.0 + a and a
+                    // difference is -0 only from -0), so the updates need
+                    // no test
+                    bool fin = true;
+#pragma unroll
+                    for (int k = 0; k < DENSE_NB_MAX; ++k) fin = fin && __builtin_isfinite(u[k]);
+                    if (fin) {
+                        // DENSE_ROWS rows' loads in flight per lane
+                        for (; r + DENSE_ROWS - 1 < r1; r += DENSE_ROWS) {
+                            double a[DENSE_ROWS];
+#pragma unroll
+                            for (int t = 0; t < DENSE_ROWS; ++t) a[t] = M[(size_t)(i0 + r + t) * n + c];
+#pragma unroll
+                            for (int k = 0; k < DENSE_NB_MAX; ++k) {
+                                if (k < w) {
+#pragma unroll
+                                    for (int t = 0; t < DENSE_ROWS; ++t) a[t] = a[t] - pan[(r + t) * w + k] * u[k];
+                                }
+                            }
+#pragma unroll
+                            for (int t = 0; t < DENSE_ROWS; ++t) M[(size_t)(i0 + r + t) * n + c] = a[t];
+                        }
+                    }
                     for (; r + 3 < r1; r += 4) {
                         double a[4];
 #pragma unroll

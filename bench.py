@@ -413,16 +413,20 @@ def logic_workload(a):
             tu = bt.autotune(precision=prec, reps=2)
             tune[key] = {"s": time.perf_counter() - t0, "factor_solve_ms_before": tu["ms_before"],
                          "factor_solve_ms_levels": tu["ms_levels"], "factor_solve_ms_persistent": tu["ms_persistent"],
-                         "persistent": tu["persistent"], "factor_solve_ms_after": tu["ms_after"]}
+                         "persistent": tu["persistent"], "factor_solve_ms_after": tu["ms_after"],
+                         "factor_solve_ms_graph": tu["ms_graph"], "factor_solve_ms_graph_fanout": tu["ms_graph_fanout"],
+                         "fanout": tu["fanout"]}
             log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution "
-                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f})")
+                f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f}; graph {tu['ms_graph']:.2f}, "
+                f"fanned out {tu['ms_graph_fanout']:.2f})")
     for _ in range(a.warmup):
-        bt.solve_dc()
-    rate, ms, kst, st, r = timed_solves(bt, a.steps)
+        bt.solve_dc(use_graph=a.graph)
+    rate, ms, kst, st, r = timed_solves(bt, a.steps, use_graph=a.graph)
     V, it = bt.voltages(), st["iterations"]
     ok64 = (st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)
-    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE)            # warm-up
-    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE)
+    bt.solve_dc(precision=_abi.NR_PREC_FP32_REFINE, use_graph=a.graph)            # warm-up
+    rate32, ms32, kst32, st32, r32 = timed_solves(bt, a.steps, precision=_abi.NR_PREC_FP32_REFINE,
+                                                  use_graph=a.graph)
     V32 = bt.voltages()
     ok32 = (st32["status"] == _abi.NR_STATUS_CONVERGED) & (st32["kcl"] <= 1e-9)
     both = ok64 & ok32
@@ -468,8 +472,8 @@ def logic_workload(a):
                                f"backward-Euler transient, FP64 and FP32 + FP64 refinement",
                    "netlist": kw, "instances": B, "nodes": T.info["n"], "mosfets": T.info["n_mos"],
                    "nnz_a": T.info["nnz_a"], "nnz_f": T.info["nnz_f"], "treewidth": T.info["treewidth"],
-                   "symbolic_ms": symbolic_ms, "batched_sweeps_per_step": r["batched_iterations"],
-                   "supernodes": T.info["n_supernodes"], "levels": T.info["n_levels"]},
+                   "symbolic_ms": symbolic_ms, "sweeps_from_hip_graphs": bool(a.graph),
+                   "batched_sweeps_per_step": r["batched_iterations"], "supernodes": T.info["n_supernodes"], "levels": T.info["n_levels"]},
         "autotune": tune or None,
         "converged_fp64": int(ok64.sum()),
         "nr_iters_per_solve": {"plain": iter_stats(it[ok64]) if ok64.any() else None,
@@ -802,6 +806,9 @@ def mixed_workload(a):
         t0 = time.perf_counter()
         if not a.no_tune:
             j["bt"].autotune(precision=prec, reps=2)
+        # one stream per level: with the level fan-out, sweeps captured from
+        # several host threads at once saw captures invalidated
+        j["bt"].set_fanout(False, precision=prec)
         tune_s += time.perf_counter() - t0
         log(f"set up {fams[j['fam']]} n={j['T'].info['n']} B={j['cnt']}: symbolic {j['symbolic_s'] * 1e3:.0f} ms, "
             f"tuned in {time.perf_counter() - t0:.1f} s")
@@ -1150,7 +1157,7 @@ def main():
     ap.add_argument("--hightw-nodes", type=int, default=2000)
     ap.add_argument("--tran-steps", type=int, default=500)
     ap.add_argument("--tran-graph", type=int, default=1, help="replay transient sweeps from hipGraphs")
-    ap.add_argument("--graph", type=int, default=1, help="mesh workload: replay DC sweeps from hipGraphs")
+    ap.add_argument("--graph", type=int, default=1, help="mesh and logic workloads: replay DC sweeps from hipGraphs")
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--seeder-probe", action="store_true", help="the seeder's products at the learned-seeder shape")
     ap.add_argument("--no-tune", action="store_true", help="default launch classes (no table, no tuning)")

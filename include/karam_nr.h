@@ -458,7 +458,10 @@ NR_API int nr_batch_set_persistent(nr_batch* batch, int32_t precision, int32_t o
  * are independent).  Off while per-kernel events are recorded
  * (record_timing = 2).  Kept by the topology, FP64 and FP32 separately;
  * AC and noise keep one stream.  nr_batch_autotune times it and keeps the
- * faster. */
+ * faster.  Batches solving with graph replays from several host threads at
+ * once (one context each) should keep it off: there, captured sweeps with
+ * the fan-out were invalidated by the runtime now and then
+ * (NR_ERR_HIP from the solve). */
 NR_API int nr_batch_set_fanout(nr_batch* batch, int32_t precision, int32_t on);
 
 /* Transient of every instance (the restated stepTransient): capacitor

@@ -68,6 +68,7 @@ constexpr int FAN_WAYS = 4;
 struct nr_ctx {
     int device = 0;
     int verbose = 0;
+    int cus = 0;                    // compute units of the device
     hipStream_t stream = nullptr;
     // side streams that fork from `stream` and join it again around the
     // launches of one level (default priority, created with the context)
@@ -2587,11 +2588,16 @@ __device__ void diag_record(const DevBatch& P, int b, const InstState& st, doubl
     P.diag[b] = d;
 }
 
-__global__ __launch_bounds__(256) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
-                                                int nsteps, int tran) {
+// NT threads per instance: 256, or 1024 when few instances are active (the
+// audits' loops over devices and nodes stride by the block; every node's
+// sum is one thread's, a hub's is thread 0's in list order, so the results
+// do not depend on NT)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_update(DevTopo T, DevBatch P, const int* __restrict__ act, int max_iter,
+                                               int nsteps, int tran) {
 #pragma clang fp contract(off)
-    __shared__ double red[8];
-    __shared__ int redi[8];
+    __shared__ double red[NT / 64];
+    __shared__ int redi[NT / 64];
     if ((int)blockIdx.x >= *P.nact) return;
     const int b = act[blockIdx.x];
     const int n = T.n;

@@ -11,8 +11,6 @@ hipGraph sweeps, as the benchmark runs them.
 - Sampled instances equal the oracle's FP32 path: bit for bit on the MOSFET
   circuits (no exp / log), within 1e-9 relative + 1e-12 V with equal
   iteration counts on the diode meshes.
-- Half of the circuits run with the level fan-out (nr_batch_set_fanout), so
-  graphs with parallel branches are captured while other threads capture.
 """
 from __future__ import annotations
 
@@ -52,9 +50,6 @@ def test_mixed_sizes_concurrently():
         f = netlists.mc_fields(nl, B, seed=500 + k, **kw)
         bt = nr.Batch(ctx, T, B)
         bt.set_fields(f)
-        # every other circuit with each level's launches on several streams
-        # (parallel graph branches captured while other threads capture too)
-        bt.set_fanout(k % 2 == 0, precision=PREC)
         runs.append({"name": name, "nl": nl, "B": B, "f": f, "T": T, "bt": bt, "w": k % 3, "ctx": ctx})
     print("mixed: " + ", ".join(f"{r['name']} n={r['T'].info['n']} B={r['B']}" for r in runs), flush=True)
 

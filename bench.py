@@ -415,7 +415,8 @@ def logic_workload(a):
                          "factor_solve_ms_levels": tu["ms_levels"], "factor_solve_ms_persistent": tu["ms_persistent"],
                          "persistent": tu["persistent"], "factor_solve_ms_after": tu["ms_after"],
                          "factor_solve_ms_graph": tu["ms_graph"], "factor_solve_ms_graph_fanout": tu["ms_graph_fanout"],
-                         "fanout": tu["fanout"]}
+                         "fanout": tu["fanout"], "few_instances_ms_before": tu["ms_few_before"],
+                         "few_instances_ms_after": tu["ms_few_after"], "few_instances_map_up_to": tu["few_max"]}
             log(f"autotune {key}: {tu['ms_before']:.2f} -> {tu['ms_after']:.2f} ms per factor + back substitution "
                 f"(per-level {tu['ms_levels']:.2f}, whole-phase {tu['ms_persistent']:.2f}; graph {tu['ms_graph']:.2f}, "
                 f"fanned out {tu['ms_graph_fanout']:.2f})")
@@ -970,7 +971,8 @@ def seeder_workload(a):
     T = nr.Topology(nl, ctx)
     gen = nr.Batch(ctx, T, B)
     tune = gen.autotune(reps=2)
-    log(f"seeder workload: logic n={n}, autotune persistent={tune['persistent']}")
+    log(f"seeder workload: logic n={n}, autotune persistent={tune['persistent']} fanout={tune['fanout']} "
+        f"few-instance map up to {tune['few_max']} ({tune['ms_few_before']:.2f} -> {tune['ms_few_after']:.2f} ms)")
     s = nr.Seeder(ctx, SEEDER_FEATURES, n, hidden=SEEDER_HIDDEN, n_hidden=SEEDER_LAYERS, init_seed=1)
 
     def make(seed):
@@ -1125,6 +1127,8 @@ def seeder_workload(a):
                    "final_loss_normalised": loss, "predict_ms_2048": pred_s * 1e3,
                    "predict_tflops_wall_incl_host_copies": fwd * B / pred_s / 1e12, "mfma_peak_tflops_bf16": 2500.0},
         "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst), "cpu_baseline": None,
+        "autotune": {k: tune[k] for k in ("persistent", "fanout", "ms_graph", "ms_graph_fanout", "few_max",
+                                          "ms_few_before", "ms_few_after")},
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -405,7 +405,11 @@ NR_API void nr_transient_opts_default(nr_transient_opts* opts);
  * (nr_batch_set_persistent) against the tuned per-level launches, and, when
  * those stay, the per-level launches replayed from a hipGraph on one stream
  * against the level fan-out (nr_batch_set_fanout), keeping the faster each
- * time.  It does not change any result: the elimination performs the same
+ * time.  Last, on batches of at least 64 instances, it tunes a second class
+ * map on graph replays of 32 instances (latency rather than throughput: the
+ * tail of a solve, when few instances are left), used for every active count
+ * up to the largest power of two where it is still the faster (few_max).
+ * It does not change any result: the elimination performs the same
  * operations in the same order whatever the team size and launch mode.
  * Overwrites the batch's solution state; not to be run while a batch of the
  * same topology is solving. */
@@ -424,6 +428,9 @@ typedef struct nr_autotune_result {
     double ms_graph;                       /* per-level launches replayed from a hipGraph, one stream (-1: not timed) */
     double ms_graph_fanout;                /* the same with each level's launches on several streams */
     int32_t fanout;                        /* 1: the level fan-out was kept (faster) */
+    double ms_few_before, ms_few_after;    /* graph replay of 32 instances with the first map, then
+                                              with the classes tuned for them (-1: not tuned) */
+    int32_t few_max;                       /* active counts up to this use the second map (0: none) */
 } nr_autotune_result;
 NR_API int nr_batch_autotune(nr_batch* batch, int32_t precision, int32_t reps,
                              nr_autotune_result* out);

@@ -195,7 +195,8 @@ class NrAutotuneResult(C.Structure):
                 ("fronts", C.c_int32 * NR_TUNE_BUCKETS), ("full", C.c_int32 * NR_TUNE_BUCKETS),
                 ("ms_before", C.c_double), ("ms_after", C.c_double), ("ms_levels", C.c_double),
                 ("ms_persistent", C.c_double), ("persistent", C.c_int32), ("ms_graph", C.c_double),
-                ("ms_graph_fanout", C.c_double), ("fanout", C.c_int32)]
+                ("ms_graph_fanout", C.c_double), ("fanout", C.c_int32), ("ms_few_before", C.c_double),
+                ("ms_few_after", C.c_double), ("few_max", C.c_int32)]
 
 
 class NrTiming(C.Structure):

@@ -449,7 +449,8 @@ class Batch:
                              "full": r.full[b]} for b in range(r.n_buckets)],
                 "ms_before": r.ms_before, "ms_after": r.ms_after, "ms_levels": r.ms_levels,
                 "ms_persistent": r.ms_persistent, "persistent": r.persistent, "ms_graph": r.ms_graph,
-                "ms_graph_fanout": r.ms_graph_fanout, "fanout": r.fanout}
+                "ms_graph_fanout": r.ms_graph_fanout, "fanout": r.fanout, "ms_few_before": r.ms_few_before,
+                "ms_few_after": r.ms_few_after, "few_max": r.few_max}
 
     def waveform(self):
         """Probe voltages of the last transient's waveform ring,

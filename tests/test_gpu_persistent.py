@@ -127,6 +127,12 @@ def test_autotune_times_the_persistent_kernels(ctx):
     if not tune["persistent"]:
         assert tune["ms_graph"] > 0 and tune["ms_graph_fanout"] > 0
         assert tune["fanout"] == int(tune["ms_graph_fanout"] < 0.99 * tune["ms_graph"])
+        # the second class map, tuned on 32 instances, serves the solve's tail
+        assert tune["ms_few_before"] > 0 and tune["ms_few_after"] > 0
+        assert tune["few_max"] in (0, 32, 64)
+        assert (tune["few_max"] > 0) == (tune["ms_few_after"] < 0.99 * tune["ms_few_before"])
+    print(f"few-instance map up to {tune['few_max']}: {tune['ms_few_before']:.2f} -> {tune['ms_few_after']:.2f} ms",
+          flush=True)
     bt.solve_dc(use_graph=1)
     _same(ref, bt)
     bt.solve_dc()

@@ -934,6 +934,71 @@ def mixed_cpu_baseline(a, jobs, prec):
     return base, parity
 
 
+def pipelined_legs(a, nl, n, s, make, mean_seed):
+    """Throughput over a stream of Monte-Carlo batches: a.seeder_pipe_batches
+    batches of 2048 test instances (seeds disjoint from the training set and
+    the single-batch legs), dealt round robin to a.seeder_pipe_streams
+    contexts (one HIP stream and host thread each, every context with its
+    own topology and tuned launch classes), so that one batch's tail of slow
+    instances overlaps the next batch's bulk.  The plain FP64 start and the
+    MLP-seeded FP64 start (seed predicted inside the timed region, plain
+    start as fallback) run the same batches the same way; value = counted
+    (KCL-converged) solves over the wall time of all batches.  The level
+    fan-out stays off: sweeps are captured from several threads."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from karam_amd import _abi, nr
+    B, S, K = 2048, a.seeder_pipe_streams, a.seeder_pipe_batches
+    data = [make(2_000_000 + 10_000 * k) for k in range(K)]
+    ctxs = [nr.Context(0) for _ in range(S)]
+    bts = []
+    for c in ctxs:
+        T = nr.Topology(nl, c)
+        bt = nr.Batch(c, T, B)
+        bt.set_fields(data[0][0])
+        bt.autotune(reps=2)
+        bt.set_fanout(False)
+        bts.append(bt)
+    lock = threading.Lock()
+
+    def run(seeded, warm=False):
+        conv = [0] * S
+
+        def work(w):
+            bt = bts[w]
+            for k in (range(w, w + 1) if warm else range(w, K, S)):
+                f, X = data[k % K]
+                bt.set_fields(f)
+                if seeded:
+                    with lock:                 # one seeder, its workspace shared
+                        V0 = s.predict_seed(X)
+                    bt.set_seed(V0)
+                    bt.solve_dc(seeded_start_step=18, polish=1, use_graph=1)
+                else:
+                    bt.clear_seed()
+                    bt.solve_dc(use_graph=1)
+                st = bt.stats()
+                conv[w] += int(((st["status"] == _abi.NR_STATUS_CONVERGED) & (st["kcl"] <= 1e-9)).sum())
+            return w
+
+        t0 = time.perf_counter()
+        with ThreadPoolExecutor(max_workers=S) as ex:
+            list(ex.map(work, range(S)))
+        wall = time.perf_counter() - t0
+        return {"value": sum(conv) / wall, "unit": "solves/s", "wall_s": wall, "counted_instances": sum(conv)}
+
+    run(False, warm=True)                      # untimed: every bucket's sweep graph of each leg
+    plain = run(False)
+    run(True, warm=True)
+    seeded = run(True)
+    log(f"pipelined over {K} batches on {S} streams: plain {plain['value']:.0f} solves/s, "
+        f"seeded {seeded['value']:.0f} solves/s")
+    for bt in bts:
+        bt.close()
+    return {"batches": K, "instances_per_batch": B, "streams": S, "plain_fp64": plain, "seeded_fp64": seeded,
+            "speedup_seeded_over_plain": seeded["value"] / plain["value"] if plain["value"] else None}
+
+
 def seeder_workload(a):
     """The learned seeder on the random-logic configuration (LOGIC20K,
     20,000 nodes): an MLP 512 -> 3 x 1024 (GELU) -> 20,000 node voltages,
@@ -1101,6 +1166,7 @@ def seeder_workload(a):
     for _ in range(5):
         s.predict_seed(X)
     pred_s = (time.perf_counter() - t1) / 5
+    pipelined = pipelined_legs(a, nl, n, s, make, mean_seed) if a.seeder_pipe_batches > 0 else None
     out = {
         "metric": METRIC, "value": legs["seeded_fp32_refine"]["value"], "unit": "solves/s", "n_gpus": 1,
         "steps": a.steps, "warmup": 1, "ms_per_step": legs["seeded_fp32_refine"]["ms_per_step"],
@@ -1129,6 +1195,7 @@ def seeder_workload(a):
         "roofline": roofline(kst, traffic=False), "kernels": kernel_table(kst), "cpu_baseline": None,
         "autotune": {k: tune[k] for k in ("persistent", "fanout", "ms_graph", "ms_graph_fanout", "few_max",
                                           "ms_few_before", "ms_few_after")},
+        "pipelined": pipelined,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -1149,6 +1216,10 @@ def main():
     ap.add_argument("--no-seeder", action="store_true")
     ap.add_argument("--seeder-train", type=int, default=8192)
     ap.add_argument("--seeder-epochs", type=int, default=20)
+    ap.add_argument("--seeder-pipe-batches", type=int, default=0,
+                    help="seeder workload: Monte-Carlo batches of the pipelined legs (0: none)")
+    ap.add_argument("--seeder-pipe-streams", type=int, default=2,
+                    help="seeder workload: contexts (HIP streams) of the pipelined legs")
     ap.add_argument("--workload", choices=["rdmesh64", "logic", "mixed", "ac", "noise", "hightw", "seeder"],
                     default="rdmesh64")
     ap.add_argument("--ac-points", type=int, default=50)

@@ -465,10 +465,11 @@ NR_API int nr_batch_set_persistent(nr_batch* batch, int32_t precision, int32_t o
  * are independent).  Off while per-kernel events are recorded
  * (record_timing = 2).  Kept by the topology, FP64 and FP32 separately;
  * AC and noise keep one stream.  nr_batch_autotune times it and keeps the
- * faster.  Batches solving with graph replays from several host threads at
- * once (one context each) should keep it off: there, captured sweeps with
- * the fan-out were invalidated by the runtime now and then
- * (NR_ERR_HIP from the solve). */
+ * faster.  The library serialises graph captures with its own device
+ * allocations and synchronous copies (one process-wide lock); a caller that
+ * allocates or copies synchronously from another thread while batches solve
+ * with graph replays can still see a capture invalidated (NR_ERR_HIP from
+ * the solve), more often with the fan-out on. */
 NR_API int nr_batch_set_fanout(nr_batch* batch, int32_t precision, int32_t on);
 
 /* Transient of every instance (the restated stepTransient): capacitor

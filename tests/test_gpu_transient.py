@@ -79,12 +79,16 @@ def test_diode_mesh_with_capacitors(ctx, O):
     assert err.max() <= 1.0
 
 
-def test_transient_from_dc_operating_point(ctx, O):
+@pytest.mark.parametrize("stages, steps", [(8, 40), (16, 200)])
+def test_transient_from_dc_operating_point(ctx, O, stages, steps):
     """'spice' start: the DC operating point (DC topology) seeds the
-    transient (transient topology) of the same instances."""
+    transient (transient topology) of the same instances; inv16 for 200
+    steps of 1 ns is the single-instance configuration's run, here with
+    Monte-Carlo instances."""
     from karam_amd import nr
-    nl = netlists.inverter_chain(8)
+    nl = netlists.inverter_chain(stages)
     B = 3
+    print(f"transient from DC: inv{stages}, {steps} steps", flush=True)
     f = netlists.mc_fields(nl, B, seed=2, sigma_vto=0.02, sigma_kp=0.02)
     D = nr.Batch(ctx, nr.Topology(nl, ctx), B)
     D.set_fields(f)
@@ -94,12 +98,14 @@ def test_transient_from_dc_operating_point(ctx, O):
     bt.set_fields(f)
     bt.set_seed(v0)
     probes = list(range(1, nl.num_nodes + 1))
-    bt.transient(h=1e-9, steps=40, start=_abi.NR_TRAN_START_SEED, probes=probes)
-    W = bt.waveform()
-    _, Wo, so = O.solve_transient_batch(nl, O.Topology(nl, transient=True), f, B, h=1e-9, steps=40, start=1,
+    bt.transient(h=1e-9, steps=steps, start=_abi.NR_TRAN_START_SEED, probes=probes)
+    W, st = bt.waveform(), bt.stats()
+    _, Wo, so = O.solve_transient_batch(nl, O.Topology(nl, transient=True), f, B, h=1e-9, steps=steps, start=1,
                                         v0=v0, probes=probes)
     np.testing.assert_array_equal(W, Wo)
     np.testing.assert_array_equal(W[:, 0, :], v0)
+    np.testing.assert_array_equal(st["iterations"], so["iterations"])
+    np.testing.assert_array_equal(st["status"], so["status"])
 
 
 def test_kind_mismatch_is_an_error(ctx):

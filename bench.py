@@ -807,8 +807,8 @@ def mixed_workload(a):
         t0 = time.perf_counter()
         if not a.no_tune:
             j["bt"].autotune(precision=prec, reps=2)
-        # one stream per level: with the level fan-out, sweeps captured from
-        # several host threads at once saw captures invalidated
+        # one stream per level: four contexts already solve concurrently, and
+        # with the level fan-out on as well the workload ran slower
         j["bt"].set_fanout(False, precision=prec)
         tune_s += time.perf_counter() - t0
         log(f"set up {fams[j['fam']]} n={j['T'].info['n']} B={j['cnt']}: symbolic {j['symbolic_s'] * 1e3:.0f} ms, "
